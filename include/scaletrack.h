@@ -1,0 +1,161 @@
+/*
+ * scaletrack.h -- C-ABI of the MI355X (gfx950) Lagrangian parcel step.
+ *
+ * The library replaces the device side of SCALE-TRACK's tracker: the per-chunk
+ * kernel launch `kernel(chunk, eulerian, mesh, executor; threads, blocks)`
+ * (run/icoJuliaParcelFoam/cavity3D/asyncParallelTracking.jl:1098-1125, compiled at
+ * :963-972), the kernel itself `evolve_on_device!`/`evolve_particle!` (:1257-1365),
+ * the bounding-box epilogue (:749-797, :1245-1255), the UTrans `fill!` (:1064-1067)
+ * and the U/UTrans `copyto!` calls (:1076-1083, :1148-1152), and the synchronous
+ * driver of syncSerialTracking.jl:325-345.
+ *
+ * Conventions: every function returns ST_OK (0) or a negative st_status; nothing
+ * aborts or throws across the ABI; st_last_error() gives the calling thread's last
+ * message.  The caller owns host arrays; the library owns device buffers behind
+ * opaque handles.  Work is queued on the context's HIP stream; functions that return
+ * host data synchronise that stream.  A handle must not be used from two threads at
+ * once.  Cell fields are AoS xyz per cell, cells numbered x fastest (the reference's
+ * VectorField, asyncParallelTracking.jl:186-205, and OpenFOAM's blockMesh order).
+ */
+#ifndef SCALETRACK_H
+#define SCALETRACK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ST_ABI_VERSION 1
+
+typedef enum st_status {
+    ST_OK = 0,
+    ST_E_INVALID = -1,     /* bad argument: null pointer, size, enum, non-finite value */
+    ST_E_DEVICE = -2,      /* HIP runtime error (no device, launch failure, ...) */
+    ST_E_NOMEM = -3,       /* device or host allocation failed */
+    ST_E_STATE = -4,       /* call out of order (e.g. evolve before upload) */
+    ST_E_UNSUPPORTED = -5, /* valid request this build does not provide */
+    ST_E_RANGE = -6        /* parcel outside the mesh, or size above a limit */
+} st_status;
+
+/* Precision forms.  ST_FP64: parcel state, fluid velocity, drag, face test and source
+ * all in double (the reference with its `scalar` alias, :172, set to Float64).
+ * ST_FP32: parcel state, fluid velocity and drag in float; the face-crossing test
+ * (x - origin) * rDelta and the momentum source in double. */
+typedef enum st_precision { ST_FP64 = 0, ST_FP32 = 1 } st_precision;
+
+/* ST_WALL: the reference's specular bounce (:1306-1335).  ST_PERIODIC: extension. */
+typedef enum st_boundary { ST_WALL = 0, ST_PERIODIC = 1 } st_boundary;
+
+/* ST_METHOD_PLAIN: reference-shaped -- one thread per parcel in the caller's order,
+ * cell velocity gathered from HBM, three FP64 atomics per deposit.
+ * ST_METHOD_SORTED: parcels kept sorted by mesh brick; one workgroup per brick stages
+ * the brick's cell velocities and sources in LDS and writes the source once. */
+typedef enum st_method { ST_METHOD_PLAIN = 0, ST_METHOD_SORTED = 1 } st_method;
+
+/* ST_FLUSH_OWNER: a brick's source is added with plain read-modify-write by the one
+ * workgroup owning it; deposits from parcels that drifted out of their brick go to a
+ * second FP64 array with atomics, merged when the source is read.
+ * ST_FLUSH_ATOMIC: every brick adds its LDS sums with FP64 atomics into one array. */
+typedef enum st_flush { ST_FLUSH_OWNER = 0, ST_FLUSH_ATOMIC = 1 } st_flush;
+
+typedef struct st_ctx st_ctx;
+typedef struct st_cloud st_cloud;
+typedef struct st_field st_field;
+typedef struct st_graph st_graph;
+
+/* Uniform hexahedral box (the reference's Mesh, :222-270, decomposition (1,1,1)). */
+typedef struct st_mesh {
+    int32_t n[3];
+    double origin[3];
+    double ending[3];
+} st_mesh;
+
+/* Per-call options.  st_params_default() fills the defaults noted here. */
+typedef struct st_params {
+    int32_t boundary;          /* st_boundary, default ST_WALL */
+    int32_t method;            /* st_method, default ST_METHOD_SORTED */
+    int32_t flush;             /* st_flush, default ST_FLUSH_OWNER */
+    int32_t substeps_per_pass; /* substeps fused in registers per launch; 0 = all nsub */
+    double cd;                 /* drag-coefficient multiplier, default 1 (reference drag) */
+    int32_t brick[3];          /* cells per brick (sorted method); default 8,8,8 */
+    int32_t block;             /* threads per workgroup; default 256 */
+    double resort_fraction;    /* re-sort when the share of parcels outside their brick in
+                                  the last pass exceeds this; default 0.08 */
+} st_params;
+
+/* ---- library ---------------------------------------------------------------- */
+int st_version(void);                     /* ST_ABI_VERSION */
+const char *st_last_error(void);          /* thread-local; "" when none */
+void st_params_default(st_params *p);
+int st_device_count(int32_t *count);
+
+/* ---- context: one device, one stream --------------------------------------- */
+int st_ctx_create(int32_t device, st_ctx **out);
+int st_ctx_destroy(st_ctx *ctx);
+int st_ctx_synchronize(st_ctx *ctx);
+
+/* ---- parcel cloud: the reference's Chunk (:329-364) ------------------------- */
+/* n parcels; mu_c, rho_d and rho_d_by_rho_c are the chunk scalars of
+ * allocate_chunk(executor, n, mu_c, rho_d, rho_d/rho_c).  per_parcel_rho != 0 adds a
+ * per-parcel density array (extension) with rho_c used for the density ratio. */
+int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, double rho_d,
+                    double rho_d_by_rho_c, double rho_c, int32_t per_parcel_rho,
+                    st_cloud **out);
+int st_cloud_destroy(st_cloud *cloud);
+/* Host SoA in the cloud's precision (double or float).  rho may be NULL unless the
+ * cloud was created with per_parcel_rho. */
+int st_cloud_upload(st_cloud *cloud, const void *d, const void *x, const void *y,
+                    const void *z, const void *u, const void *v, const void *w,
+                    const void *rho);
+/* Returns parcels in the order they were uploaded, whatever the device order. */
+int st_cloud_download(st_cloud *cloud, void *d, void *x, void *y, void *z, void *u,
+                      void *v, void *w);
+int st_cloud_count(const st_cloud *cloud, int64_t *n);
+
+/* ---- Eulerian fields: the reference's TwoWayEulerian (:207-220) ------------- */
+int st_field_create(st_ctx *ctx, const st_mesh *mesh, int32_t precision, st_field **out);
+int st_field_destroy(st_field *field);
+/* U: host AoS xyz, ncell*3 elements of elem_bytes (4 or 8) each; converted to the
+ * field's precision. */
+int st_field_upload_U(st_field *field, const void *U, int32_t elem_bytes);
+/* UTrans = 0 (asyncParallelTracking.jl:1064-1067, syncSerialTracking.jl:553). */
+int st_field_zero_source(st_field *field);
+/* UTrans: host AoS xyz, ncell*3 elements of elem_bytes (8 = the FP64 sums, 4 =
+ * rounded to float for a single-precision caller). */
+int st_field_download_source(st_field *field, void *out, int32_t elem_bytes);
+
+/* ---- the parcel step -------------------------------------------------------- */
+/* Advances every parcel of `cloud` by nsub substeps of dt_fluid / nsub (the
+ * reference's nSteps = 10 and time[1].Delta_t / nSteps, :1260-1261) through the
+ * frozen cell velocity of `field`, adding the momentum source
+ * sum (m_d / rho_c) (v_in - v_out) per cell into the field's UTrans.  Queued on the
+ * context stream.  bbox_out (may be NULL) receives min xyz, max xyz of the new
+ * positions, and then the call synchronises.  params may be NULL (defaults). */
+int st_evolve(st_ctx *ctx, st_cloud *cloud, st_field *field, double dt_fluid, int32_t nsub,
+              const st_params *params, double bbox_out[6]);
+
+/* Share of parcels that were outside their sorted brick during the last sorted pass
+ * (synchronises); -1 if unknown. */
+int st_cloud_stray_fraction(st_cloud *cloud, double *fraction);
+/* Forces a sort of the cloud by brick of `field`'s mesh before the next step. */
+int st_cloud_sort(st_ctx *ctx, st_cloud *cloud, st_field *field, const st_params *params);
+
+/* ---- HIP graph of one fluid step -------------------------------------------- */
+/* Captures zeroing UTrans plus the nsub substeps as one hipGraph (sorted method: the
+ * cloud is sorted first, outside the graph; no re-sort happens inside it). */
+int st_graph_capture(st_ctx *ctx, st_cloud *cloud, st_field *field, double dt_fluid,
+                     int32_t nsub, const st_params *params, int32_t zero_source,
+                     st_graph **out);
+int st_graph_launch(st_graph *graph);
+int st_graph_destroy(st_graph *graph);
+
+/* ---- timing helper ---------------------------------------------------------- */
+/* Average device time (ms) of the last st_evolve's dominant kernel launches, from HIP
+ * events recorded on the context stream around them; -1 if not recorded. */
+int st_last_kernel_ms(st_ctx *ctx, double *ms, int32_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,317 @@
+/*
+ * st_oracle.cpp -- CPU restatement of SCALE-TRACK's Lagrangian parcel step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see st_oracle.h): the checker that the HIP library is
+ * compared against, and the CPU baseline that bench.py times.  Never linked into
+ * the product.
+ *
+ * Written as C++17 (built with g++, -ffp-contract=off) only so that one templated
+ * step can be instantiated for double, for float state with double face test and
+ * source, and for forward-mode dual numbers (the gradient check).  Operation order
+ * follows the reference's Julia expressions literally, because Julia evaluates
+ * left to right and never contracts a*b+c into an FMA without @fastmath/@muladd:
+ *   m    = ((rho_d/rho_c * pi) * d^3) / 6          asyncParallelTracking.jl:1280
+ *   f    = ((18 * mu) * dt) / (rho * (d*d))        asyncParallelTracking.jl:1294
+ *   v'   = (v + f*u) / (1 + f)                     asyncParallelTracking.jl:1295
+ *   x'   = x + v' * dt                             asyncParallelTracking.jl:1296
+ * d^3 and d^2 are Julia literal_pow, i.e. (d*d)*d and d*d.
+ */
+#include "st_oracle.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <array>
+#include <limits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+const double kPi = 3.14159265358979323846;  /* Float64(pi), correctly rounded */
+
+/* Uniform box mesh, asyncParallelTracking.jl:222-270 (construct_mesh):
+ * L = ending - origin, Delta = L / N, rDelta = 1 / Delta, all in the face-test
+ * precision (double here).  Decomposition (1,1,1): partitionN = N. */
+struct MeshD {
+    int32_t n[3];
+    int64_t nxny;
+    double o[3], L[3], delta[3], rdelta[3];
+    explicit MeshD(const int32_t nn[3], const double org[3], const double end[3]) {
+        for (int a = 0; a < 3; ++a) {
+            n[a] = nn[a];
+            o[a] = org[a];
+            L[a] = end[a] - org[a];
+            delta[a] = L[a] / (double)nn[a];
+            rdelta[a] = 1.0 / delta[a];
+        }
+        nxny = (int64_t)n[0] * n[1];
+    }
+};
+
+/* locate_ijk, asyncParallelTracking.jl:663-677: floor((x - o) * rDelta) per axis
+ * (IEEE floor, never truncation -- the reference's comment at :664-666 explains
+ * why unsafe_trunc is wrong for -1 < n < 0). */
+inline int32_t axis_index(double xa, double o, double rd) {
+    return (int32_t)std::floor((xa - o) * rd);
+}
+
+struct Loc {
+    int32_t ijk[3];
+    int64_t cell;  /* 0-based folded linear index (the reference's posI - 1) */
+    int64_t part;  /* 0-based folded partition index (partitionI - 1) */
+};
+
+/* locate, asyncParallelTracking.jl:679-697, for decomposition (1,1,1):
+ * divrem(I, N) truncates toward zero (as C's / and %); the linear indices fold the
+ * local and partition indices with abs(), x fastest. */
+inline Loc locate_wall(const MeshD &m, double x, double y, double z) {
+    Loc l;
+    const double p[3] = {x, y, z};
+    int64_t loc[3], par[3];
+    for (int a = 0; a < 3; ++a) {
+        l.ijk[a] = axis_index(p[a], m.o[a], m.rdelta[a]);
+        par[a] = l.ijk[a] / m.n[a];
+        loc[a] = l.ijk[a] % m.n[a];
+    }
+    l.cell = std::llabs(loc[2]) * m.nxny + std::llabs(loc[1]) * m.n[0] + std::llabs(loc[0]);
+    /* decompositionXTimesY = decomposition.x = 1 */
+    l.part = std::llabs(par[2]) + std::llabs(par[1]) + std::llabs(par[0]);
+    return l;
+}
+
+/* Periodic extension: index clamped into [0, N) (a wrapped coordinate can round onto
+ * the far face `ending`, which belongs to the last cell). */
+inline int32_t clamp_index(int32_t i, int32_t n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
+
+inline int64_t linear(const MeshD &m, const int32_t ijk[3]) {
+    return (int64_t)ijk[2] * m.nxny + (int64_t)ijk[1] * m.n[0] + ijk[0];
+}
+
+template <class S>
+struct Vec3 {
+    S c[3];
+};
+
+/* Per-call constants of one precision form. */
+template <class S>
+struct Consts {
+    S mu, rho_chunk, dt, cd, one, two, eighteen;
+    S L[3];
+};
+
+/* The step of one parcel for nsteps substeps: asyncParallelTracking.jl:1271-1365.
+ * S = state/drag type, D = face-test/source type (double). */
+template <class S>
+void evolve_particle(const st_oracle_args &a, const MeshD &m, const Consts<S> &k,
+                     int64_t i, int nsteps, double *src, double bb[6]) {
+    S *d = (S *)a.d;
+    S *X = (S *)a.x, *Y = (S *)a.y, *Z = (S *)a.z;
+    S *U = (S *)a.u, *V = (S *)a.v, *W = (S *)a.w;
+    const S *Uc = (const S *)a.U;
+    const bool periodic = a.boundary == ST_O_PERIODIC;
+
+    const S dia = d[i];
+    S rho = k.rho_chunk;
+    double rdr = a.rho_d_by_rho_c;
+    if (a.rho) {
+        rho = ((const S *)a.rho)[i];
+        rdr = (double)rho / a.rho_c;  /* per-parcel density: extension */
+    }
+    /* asyncParallelTracking.jl:1280, in the source precision */
+    const double dd = (double)dia;
+    const double mByRho = ((rdr * kPi) * ((dd * dd) * dd)) / 6.0;
+
+    Vec3<S> pos{{X[i], Y[i], Z[i]}}, vel{{U[i], V[i], W[i]}};
+    Vec3<S> posNew = pos, velNew = vel;
+
+    /* :1286-1289 */
+    int64_t posI, partI;
+    if (!periodic) {
+        Loc l = locate_wall(m, (double)pos.c[0], (double)pos.c[1], (double)pos.c[2]);
+        posI = l.cell;
+        partI = l.part;
+    } else {
+        int32_t ijk[3];
+        for (int ax = 0; ax < 3; ++ax)
+            ijk[ax] = clamp_index(axis_index((double)pos.c[ax], m.o[ax], m.rdelta[ax]), m.n[ax]);
+        posI = linear(m, ijk);
+        partI = 0;
+    }
+    Vec3<S> uc{{Uc[3 * posI], Uc[3 * posI + 1], Uc[3 * posI + 2]}};
+    double s0[3] = {(double)vel.c[0], (double)vel.c[1], (double)vel.c[2]};
+
+    for (int t = 1; t <= nsteps; ++t) {
+        /* :1294-1296 implicit Euler with Stokes drag */
+        S f = ((k.eighteen * k.mu) * k.dt) / (rho * (dia * dia));
+        f = k.cd * f;  /* drag-coefficient multiplier (extension); exact for cd = 1 */
+        const S denom = k.one + f;
+        for (int c = 0; c < 3; ++c) velNew.c[c] = (vel.c[c] + f * uc.c[c]) / denom;
+        for (int c = 0; c < 3; ++c) posNew.c[c] = pos.c[c] + velNew.c[c] * k.dt;
+
+        if (!periodic) {
+            /* :1303 */
+            Loc l = locate_wall(m, (double)posNew.c[0], (double)posNew.c[1], (double)posNew.c[2]);
+            /* :1306 */
+            if (l.cell != posI || l.part != partI) {
+                bool hit = false;
+                for (int c = 0; c < 3; ++c) {
+                    /* :1309-1311 */
+                    const bool b = (l.ijk[c] < 0) || (l.ijk[c] >= m.n[c]);
+                    if (b) {
+                        /* :1313-1329 and update_at_boundary!, :730-736 */
+                        posNew.c[c] = pos.c[c];
+                        velNew.c[c] = -vel.c[c];
+                        s0[c] = s0[c] - 2.0 * (double)vel.c[c];
+                        hit = true;
+                    }
+                }
+                /* :1333-1335 */
+                if (hit) l = locate_wall(m, (double)posNew.c[0], (double)posNew.c[1], (double)posNew.c[2]);
+                /* :1337-1340 re-reads the old partition; a no-op for one partition,
+                 * and partitionI is never updated inside the loop. */
+                /* :1342-1348 */
+                if (t != nsteps && l.cell != posI) {
+                    for (int c = 0; c < 3; ++c)
+                        src[3 * posI + c] += mByRho * (s0[c] - (double)velNew.c[c]);
+                    for (int c = 0; c < 3; ++c) s0[c] = (double)velNew.c[c];
+                    for (int c = 0; c < 3; ++c) uc.c[c] = Uc[3 * l.cell + c];
+                    posI = l.cell;
+                }
+            }
+        } else {
+            /* periodic extension: wrap each axis that left [0, N) by one period */
+            int32_t ijk[3];
+            for (int c = 0; c < 3; ++c) {
+                int32_t r = axis_index((double)posNew.c[c], m.o[c], m.rdelta[c]);
+                if (r < 0) {
+                    posNew.c[c] = posNew.c[c] + k.L[c];
+                    r = axis_index((double)posNew.c[c], m.o[c], m.rdelta[c]);
+                } else if (r >= m.n[c]) {
+                    posNew.c[c] = posNew.c[c] - k.L[c];
+                    r = axis_index((double)posNew.c[c], m.o[c], m.rdelta[c]);
+                }
+                ijk[c] = clamp_index(r, m.n[c]);
+            }
+            const int64_t cellNew = linear(m, ijk);
+            if (t != nsteps && cellNew != posI) {
+                for (int c = 0; c < 3; ++c)
+                    src[3 * posI + c] += mByRho * (s0[c] - (double)velNew.c[c]);
+                for (int c = 0; c < 3; ++c) s0[c] = (double)velNew.c[c];
+                for (int c = 0; c < 3; ++c) uc.c[c] = Uc[3 * cellNew + c];
+                posI = cellNew;
+            }
+        }
+        /* :1351-1352 */
+        vel = velNew;
+        pos = posNew;
+    }
+    /* :1355-1360 */
+    X[i] = posNew.c[0];
+    Y[i] = posNew.c[1];
+    Z[i] = posNew.c[2];
+    U[i] = velNew.c[0];
+    V[i] = velNew.c[1];
+    W[i] = velNew.c[2];
+    /* :1361 */
+    for (int c = 0; c < 3; ++c) src[3 * posI + c] += mByRho * (s0[c] - (double)velNew.c[c]);
+    /* :1362 bounding box of the new positions */
+    for (int c = 0; c < 3; ++c) {
+        const double p = (double)posNew.c[c];
+        bb[c] = std::min(bb[c], p);
+        bb[3 + c] = std::max(bb[3 + c], p);
+    }
+}
+
+template <class S>
+int evolve_all(st_oracle_args *a) {
+    MeshD m(a->n, a->origin, a->ending);
+    const int64_t ncell = (int64_t)a->n[0] * a->n[1] * a->n[2];
+    Consts<S> k;
+    k.mu = (S)a->mu_c;
+    k.rho_chunk = (S)a->rho_d;
+    /* Time stores dt as `scalar` (:272-275); the substep is dt / nSteps (:1261). */
+    k.dt = (S)a->dt_fluid / (S)a->nsub;
+    k.cd = (S)a->cd;
+    k.one = (S)1;
+    k.two = (S)2;
+    k.eighteen = (S)18;
+    for (int c = 0; c < 3; ++c) k.L[c] = (S)(a->ending[c] - a->origin[c]);
+
+    const int per = a->substeps_per_pass > 0 ? a->substeps_per_pass : a->nsub;
+    double bb[6] = {std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
+                    std::numeric_limits<double>::infinity(), -std::numeric_limits<double>::infinity(),
+                    -std::numeric_limits<double>::infinity(), -std::numeric_limits<double>::infinity()};
+    int nthreads = a->nthreads > 0 ? a->nthreads : 1;
+#ifndef _OPENMP
+    nthreads = 1;
+#endif
+    for (int done = 0; done < a->nsub; done += per) {
+        const int ns = std::min(per, a->nsub - done);
+        if (nthreads == 1) {
+            /* syncSerialTracking.jl:318-320 loop shape, with the "+=" deposit of
+             * asyncParallelTracking.jl:706 */
+            for (int64_t i = 0; i < a->np; ++i) evolve_particle<S>(*a, m, k, i, ns, a->source, bb);
+        } else {
+#ifdef _OPENMP
+            std::vector<std::vector<double>> priv(nthreads);
+            std::vector<std::array<double, 6>> pbb(nthreads);
+#pragma omp parallel num_threads(nthreads)
+            {
+                const int tid = omp_get_thread_num();
+                std::vector<double> &s = priv[tid];
+                s.assign(3 * ncell, 0.0);
+                double b[6];
+                std::memcpy(b, bb, sizeof b);
+#pragma omp for schedule(static)
+                for (int64_t i = 0; i < a->np; ++i) evolve_particle<S>(*a, m, k, i, ns, s.data(), b);
+                std::memcpy(pbb[tid].data(), b, sizeof b);
+            }
+            for (int t = 0; t < nthreads; ++t) {
+                const std::vector<double> &s = priv[t];
+                if (s.empty()) continue;
+                for (int64_t j = 0; j < 3 * ncell; ++j) a->source[j] += s[j];
+                for (int c = 0; c < 3; ++c) {
+                    bb[c] = std::min(bb[c], pbb[t][c]);
+                    bb[3 + c] = std::max(bb[3 + c], pbb[t][3 + c]);
+                }
+            }
+#endif
+        }
+    }
+    std::memcpy(a->bbox, bb, sizeof bb);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int st_oracle_evolve(st_oracle_args *a) {
+    if (!a || a->nsub <= 0 || a->np < 0) return -1;
+    for (int c = 0; c < 3; ++c)
+        if (a->n[c] <= 0 || !(a->ending[c] > a->origin[c])) return -1;
+    if (a->precision == ST_O_FP64) return evolve_all<double>(a);
+    if (a->precision == ST_O_FP32) return evolve_all<float>(a);
+    return -1;
+}
+
+extern "C" int st_oracle_locate(const int32_t n[3], const double origin[3], const double ending[3],
+                                double x, double y, double z, int32_t ijk[3], int64_t *cell,
+                                int64_t *partition) {
+    MeshD m(n, origin, ending);
+    Loc l = locate_wall(m, x, y, z);
+    for (int c = 0; c < 3; ++c) ijk[c] = l.ijk[c];
+    *cell = l.cell;
+    *partition = l.part;
+    return 0;
+}
+
+/* estimate_source!, asyncParallelTracking.jl:738-747 */
+extern "C" void st_oracle_extrapolate(double *cur, double *prev, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        cur[i] = 2.0 * cur[i] - prev[i];
+        prev[i] = cur[i];
+    }
+}

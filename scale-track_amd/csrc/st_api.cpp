@@ -1,0 +1,798 @@
+// st_api.cpp -- C-ABI of the gfx950 parcel step (declared in include/scaletrack.h).
+//
+// Replaces the device-side calls of the reference's tracker: the per-chunk launch of
+// evolve_on_device! (run/icoJuliaParcelFoam/cavity3D/asyncParallelTracking.jl:1098-1125),
+// the UTrans fill! (:1064-1067), the U/UTrans copies (:1076-1083, :1148-1152) and the
+// bounding-box readback (:1106, :1113).  No status escapes as an exception.
+#include "../../include/scaletrack.h"
+#include "st_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <new>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define ST_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(e_ == hipErrorOutOfMemory ? ST_E_NOMEM : ST_E_DEVICE, "%s: %s (%s:%d)", \
+                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                  \
+    } while (0)
+
+#define ST_TRY(call)            \
+    do {                        \
+        int r_ = (call);        \
+        if (r_ != ST_OK) return r_; \
+    } while (0)
+
+constexpr int kNF = 8;  // d, x, y, z, u, v, w, rho
+
+}  // namespace
+
+struct st_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int timed_launches = 0;  // launches between ev[0] and ev[1] of the last st_evolve
+};
+
+struct st_cloud {
+    st_ctx *ctx = nullptr;
+    int prec = ST_FP64;
+    int64_t n = 0;
+    size_t es = 8;
+    double mu_c = 0, rho_d = 0, rdr = 0, rho_c = 0;
+    int has_rho = 0;
+    void *buf[2][kNF] = {};
+    int32_t *pid[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool uploaded = false;
+    bool fresh = false;     // uploaded since the last range check
+    bool permuted = false;  // device order differs from upload order
+    // sort state
+    uint32_t *keys = nullptr;
+    uint32_t *counts = nullptr;
+    uint32_t *cursor = nullptr;
+    int32_t *brick_start = nullptr;
+    int32_t bins_cap = 0;
+    bool sorted = false;
+    int32_t sort_n[3] = {0, 0, 0}, sort_bd[3] = {0, 0, 0}, sort_periodic = -1;
+    int32_t nb[3] = {0, 0, 0};
+    int32_t nbricks = 0;
+    double sort_o[3] = {0, 0, 0}, sort_e[3] = {0, 0, 0};
+    // statistics of the last launch sequence
+    unsigned long long *bbox = nullptr;  // device, 6 keys
+    unsigned int *strays = nullptr;      // device
+    unsigned int *bad = nullptr;         // device
+    unsigned int *strays_host = nullptr; // pinned
+    hipEvent_t strays_ev = nullptr;
+    bool strays_pending = false;
+    int64_t strays_den = 0;  // parcels x passes behind strays_host
+    double last_stray_fraction = -1.0;
+};
+
+struct st_field {
+    st_ctx *ctx = nullptr;
+    st_mesh mesh{};
+    st::Geo geo{};
+    int prec = ST_FP64;
+    size_t es = 8;
+    int64_t ncell = 0;
+    void *U = nullptr;
+    double *src = nullptr;
+    double *stray = nullptr;
+    bool U_set = false;
+    bool stray_dirty = false;
+};
+
+struct st_graph {
+    st_ctx *ctx = nullptr;
+    st_field *field = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool owner_flush = false;
+};
+
+namespace {
+
+int check_ctx(st_ctx *ctx) {
+    if (!ctx) return fail(ST_E_INVALID, "null context");
+    ST_HIP(hipSetDevice(ctx->device));
+    return ST_OK;
+}
+
+bool finite_all(const void *p, int64_t n, size_t es) {
+    if (es == 8) {
+        const double *a = (const double *)p;
+        for (int64_t i = 0; i < n; ++i)
+            if (!std::isfinite(a[i])) return false;
+    } else {
+        const float *a = (const float *)p;
+        for (int64_t i = 0; i < n; ++i)
+            if (!std::isfinite(a[i])) return false;
+    }
+    return true;
+}
+
+bool positive_all(const void *p, int64_t n, size_t es) {
+    if (es == 8) {
+        const double *a = (const double *)p;
+        for (int64_t i = 0; i < n; ++i)
+            if (!(a[i] > 0)) return false;
+    } else {
+        const float *a = (const float *)p;
+        for (int64_t i = 0; i < n; ++i)
+            if (!(a[i] > 0)) return false;
+    }
+    return true;
+}
+
+st::Geo make_geo(const st_mesh &m) {
+    st::Geo g{};
+    for (int c = 0; c < 3; ++c) {
+        g.n[c] = m.n[c];
+        g.o[c] = m.origin[c];
+        const double L = m.ending[c] - m.origin[c];
+        const double delta = L / (double)m.n[c];
+        g.rd[c] = 1.0 / delta;
+    }
+    g.nxny = (uint32_t)m.n[0] * (uint32_t)m.n[1];
+    return g;
+}
+
+template <class T>
+st::Soa<T> soa(st_cloud *c, int which) {
+    st::Soa<T> s;
+    void **b = c->buf[which];
+    s.d = (T *)b[0];
+    s.x = (T *)b[1];
+    s.y = (T *)b[2];
+    s.z = (T *)b[3];
+    s.u = (T *)b[4];
+    s.v = (T *)b[5];
+    s.w = (T *)b[6];
+    s.rho = c->has_rho ? (const T *)b[7] : nullptr;
+    return s;
+}
+
+int alloc_set(st_cloud *c, int which) {
+    const size_t bytes = (size_t)(c->n > 0 ? c->n : 1) * c->es;
+    for (int f = 0; f < kNF; ++f) {
+        if (f == 7 && !c->has_rho) continue;
+        if (!c->buf[which][f]) ST_HIP(hipMalloc(&c->buf[which][f], bytes));
+    }
+    if (!c->pid[which]) ST_HIP(hipMalloc((void **)&c->pid[which], (size_t)(c->n > 0 ? c->n : 1) * 4));
+    return ST_OK;
+}
+
+int validate_params(const st_params &p) {
+    if (p.boundary != ST_WALL && p.boundary != ST_PERIODIC)
+        return fail(ST_E_INVALID, "boundary must be ST_WALL or ST_PERIODIC (got %d)", p.boundary);
+    if (p.method != ST_METHOD_PLAIN && p.method != ST_METHOD_SORTED)
+        return fail(ST_E_INVALID, "method must be ST_METHOD_PLAIN or ST_METHOD_SORTED (got %d)", p.method);
+    if (p.flush != ST_FLUSH_OWNER && p.flush != ST_FLUSH_ATOMIC)
+        return fail(ST_E_INVALID, "flush must be ST_FLUSH_OWNER or ST_FLUSH_ATOMIC (got %d)", p.flush);
+    if (p.substeps_per_pass < 0) return fail(ST_E_INVALID, "substeps_per_pass must be >= 0");
+    if (!std::isfinite(p.cd) || p.cd < 0) return fail(ST_E_INVALID, "cd must be finite and >= 0");
+    for (int c = 0; c < 3; ++c)
+        if (p.brick[c] < 1 || p.brick[c] > 64) return fail(ST_E_INVALID, "brick dims must be in [1, 64]");
+    const int64_t tc = (int64_t)p.brick[0] * p.brick[1] * p.brick[2];
+    if (tc > 2048) return fail(ST_E_INVALID, "brick holds %lld cells; at most 2048", (long long)tc);
+    if (p.block != 64 && p.block != 128 && p.block != 256 && p.block != 512 && p.block != 1024)
+        return fail(ST_E_INVALID, "block must be 64, 128, 256, 512 or 1024 (got %d)", p.block);
+    if (!(p.resort_fraction >= 0)) return fail(ST_E_INVALID, "resort_fraction must be >= 0");
+    return ST_OK;
+}
+
+template <class T>
+int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
+    hipStream_t s = c->ctx->stream;
+    int32_t nb[3];
+    int64_t nbr = 1;
+    for (int a = 0; a < 3; ++a) {
+        nb[a] = (f->mesh.n[a] + p.brick[a] - 1) / p.brick[a];
+        nbr *= nb[a];
+    }
+    if (nbr > (1 << 26)) return fail(ST_E_RANGE, "too many bricks (%lld)", (long long)nbr);
+    if (nbr > c->bins_cap) {
+        if (c->counts) ST_HIP(hipFree(c->counts));
+        if (c->cursor) ST_HIP(hipFree(c->cursor));
+        if (c->brick_start) ST_HIP(hipFree(c->brick_start));
+        c->counts = nullptr;
+        c->cursor = nullptr;
+        c->brick_start = nullptr;
+        ST_HIP(hipMalloc((void **)&c->counts, (size_t)nbr * 4));
+        ST_HIP(hipMalloc((void **)&c->cursor, (size_t)nbr * 4));
+        ST_HIP(hipMalloc((void **)&c->brick_start, (size_t)(nbr + 1) * 4));
+        c->bins_cap = (int32_t)nbr;
+    }
+    if (!c->keys) ST_HIP(hipMalloc((void **)&c->keys, (size_t)(c->n > 0 ? c->n : 1) * 4));
+    ST_TRY(alloc_set(c, 1 - c->cur));
+    const uint32_t np = (uint32_t)c->n;
+    ST_HIP(st::launch_fill_u32(c->counts, 0u, (size_t)nbr, s));
+    ST_HIP(st::launch_fill_u32(c->bad, 0u, 1, s));
+    ST_HIP(st::launch_brick_keys<T>(soa<T>(c, c->cur), np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC,
+                                    c->keys, c->counts, c->bad, s));
+    ST_HIP(st::launch_scan(c->counts, (int32_t)nbr, c->brick_start, c->cursor, s));
+    ST_HIP(st::launch_scatter<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur),
+                                 c->pid[1 - c->cur], c->keys, c->cursor, np, s));
+    c->cur = 1 - c->cur;
+    c->permuted = true;
+    c->sorted = true;
+    for (int a = 0; a < 3; ++a) {
+        c->sort_n[a] = f->mesh.n[a];
+        c->sort_bd[a] = p.brick[a];
+        c->nb[a] = nb[a];
+        c->sort_o[a] = f->mesh.origin[a];
+        c->sort_e[a] = f->mesh.ending[a];
+    }
+    c->sort_periodic = p.boundary;
+    c->nbricks = (int32_t)nbr;
+    c->strays_pending = false;
+    c->last_stray_fraction = -1.0;
+    return ST_OK;
+}
+
+bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p) {
+    if (!c->sorted || c->sort_periodic != p.boundary) return false;
+    for (int a = 0; a < 3; ++a)
+        if (c->sort_n[a] != f->mesh.n[a] || c->sort_bd[a] != p.brick[a] ||
+            c->sort_o[a] != f->mesh.origin[a] || c->sort_e[a] != f->mesh.ending[a])
+            return false;
+    return true;
+}
+
+int range_check(st_cloud *c, st_field *f) {
+    hipStream_t s = c->ctx->stream;
+    ST_HIP(st::launch_fill_u32(c->bad, 0u, 1, s));
+    if (c->prec == ST_FP64)
+        ST_HIP(st::launch_range_check<double>(soa<double>(c, c->cur), (uint32_t)c->n, f->geo, c->bad, s));
+    else
+        ST_HIP(st::launch_range_check<float>(soa<float>(c, c->cur), (uint32_t)c->n, f->geo, c->bad, s));
+    unsigned int bad = 0;
+    ST_HIP(hipMemcpyAsync(&bad, c->bad, 4, hipMemcpyDeviceToHost, s));
+    ST_HIP(hipStreamSynchronize(s));
+    if (bad) return fail(ST_E_RANGE, "%u parcels lie outside the mesh [origin, ending)", bad);
+    c->fresh = false;
+    return ST_OK;
+}
+
+template <class T>
+int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int nsub, const st_params &p,
+               bool capturing) {
+    hipStream_t s = ctx->stream;
+    st::PassArgs<T> a{};
+    a.s = soa<T>(c, c->cur);
+    a.U = (const T *)f->U;
+    a.src = f->src;
+    a.stray = f->stray;
+    a.brick_start = c->brick_start;
+    a.g = f->geo;
+    a.ph.dt = (T)dt_fluid / (T)nsub;  // Time stores dt as scalar (:272-275), then dt / nSteps (:1261)
+    a.ph.mu = (T)c->mu_c;
+    a.ph.rho = (T)c->rho_d;
+    a.ph.cd = (T)p.cd;
+    for (int k = 0; k < 3; ++k) a.ph.L[k] = (T)(f->mesh.ending[k] - f->mesh.origin[k]);
+    a.ph.rdr = c->rdr;
+    a.ph.rho_c = c->rho_c;
+    a.st.bbox = c->bbox;
+    a.st.strays = c->strays;
+    a.nbricks = c->nbricks;
+    for (int k = 0; k < 3; ++k) {
+        a.nb[k] = c->nb[k];
+        a.bd[k] = p.brick[k];
+    }
+    a.tile_cells = p.brick[0] * p.brick[1] * p.brick[2];
+    a.flush = p.flush;
+    a.np = (uint32_t)c->n;
+    const int per = p.substeps_per_pass > 0 ? p.substeps_per_pass : nsub;
+    const int periodic = p.boundary == ST_PERIODIC;
+
+    ST_HIP(st::launch_reset_stats(a.st, s));
+    if (!capturing) ST_HIP(hipEventRecord(ctx->ev[0], s));
+    int launches = 0;
+    for (int done = 0; done < nsub; done += per) {
+        a.nsteps = per < nsub - done ? per : nsub - done;
+        if (p.method == ST_METHOD_SORTED)
+            ST_HIP(st::launch_pass_sorted<T>(a, periodic, p.block, s));
+        else
+            ST_HIP(st::launch_pass_plain<T>(a, periodic, p.block, s));
+        ++launches;
+    }
+    if (!capturing) {
+        ST_HIP(hipEventRecord(ctx->ev[1], s));
+        ctx->timed_launches = launches;
+    }
+    if (p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER) f->stray_dirty = true;
+    if (p.method == ST_METHOD_SORTED && !capturing) {
+        ST_HIP(hipMemcpyAsync(c->strays_host, c->strays, 4, hipMemcpyDeviceToHost, s));
+        ST_HIP(hipEventRecord(c->strays_ev, s));
+        c->strays_pending = true;
+        c->strays_den = c->n * (int64_t)launches;
+    }
+    return ST_OK;
+}
+
+int prepare(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int nsub, const st_params *pin,
+            st_params &p) {
+    ST_TRY(check_ctx(ctx));
+    if (!c || !f) return fail(ST_E_INVALID, "null cloud or field");
+    if (c->ctx != ctx || f->ctx != ctx) return fail(ST_E_INVALID, "cloud/field belong to another context");
+    if (!std::isfinite(dt_fluid) || !(dt_fluid > 0)) return fail(ST_E_INVALID, "dt_fluid must be finite and > 0");
+    if (nsub < 1) return fail(ST_E_INVALID, "nsub must be >= 1 (got %d)", nsub);
+    if (pin)
+        p = *pin;
+    else
+        st_params_default(&p);
+    ST_TRY(validate_params(p));
+    if (c->prec != f->prec) return fail(ST_E_INVALID, "cloud and field precision differ");
+    if (!c->uploaded) return fail(ST_E_STATE, "cloud has no parcels uploaded");
+    if (!f->U_set) return fail(ST_E_STATE, "field velocity U was never uploaded");
+    if (c->fresh) ST_TRY(range_check(c, f));
+    return ST_OK;
+}
+
+int maybe_sort(st_cloud *c, st_field *f, const st_params &p) {
+    bool need = !sort_matches(c, f, p);
+    if (!need && c->strays_pending && hipEventQuery(c->strays_ev) == hipSuccess) {
+        c->strays_pending = false;
+        c->last_stray_fraction = c->strays_den > 0 ? (double)*c->strays_host / (double)c->strays_den : 0.0;
+        if (c->last_stray_fraction > p.resort_fraction) need = true;
+    }
+    if (!need) return ST_OK;
+    return c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int st_version(void) { return ST_ABI_VERSION; }
+
+const char *st_last_error(void) { return g_err.c_str(); }
+
+void st_params_default(st_params *p) {
+    if (!p) return;
+    p->boundary = ST_WALL;
+    p->method = ST_METHOD_SORTED;
+    p->flush = ST_FLUSH_OWNER;
+    p->substeps_per_pass = 0;
+    p->cd = 1.0;
+    p->brick[0] = 8;
+    p->brick[1] = 8;
+    p->brick[2] = 8;
+    p->block = 256;
+    p->resort_fraction = 0.08;
+}
+
+int st_device_count(int32_t *count) {
+    if (!count) return fail(ST_E_INVALID, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail(ST_E_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *count = n;
+    return ST_OK;
+}
+
+int st_ctx_create(int32_t device, st_ctx **out) {
+    if (!out) return fail(ST_E_INVALID, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(ST_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(ST_E_INVALID, "device %d out of range [0, %d)", device, n);
+    st_ctx *c = new (std::nothrow) st_ctx;
+    if (!c) return fail(ST_E_NOMEM, "host allocation failed");
+    c->device = device;
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev[1]);
+    if (e != hipSuccess) {
+        st_ctx_destroy(c);
+        return fail(ST_E_DEVICE, "context creation: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return ST_OK;
+}
+
+int st_ctx_destroy(st_ctx *ctx) {
+    if (!ctx) return ST_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->ev[0]) hipEventDestroy(ctx->ev[0]);
+    if (ctx->ev[1]) hipEventDestroy(ctx->ev[1]);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ST_OK;
+}
+
+int st_ctx_synchronize(st_ctx *ctx) {
+    ST_TRY(check_ctx(ctx));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    return ST_OK;
+}
+
+int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, double rho_d,
+                    double rho_d_by_rho_c, double rho_c, int32_t per_parcel_rho, st_cloud **out) {
+    if (!out) return fail(ST_E_INVALID, "null output handle");
+    *out = nullptr;
+    ST_TRY(check_ctx(ctx));
+    if (precision != ST_FP64 && precision != ST_FP32) return fail(ST_E_INVALID, "precision must be ST_FP64 or ST_FP32");
+    if (n < 0 || n > 0x7fffffffLL) return fail(ST_E_RANGE, "parcel count %lld outside [0, 2^31)", (long long)n);
+    if (!std::isfinite(mu_c) || mu_c < 0) return fail(ST_E_INVALID, "mu_c must be finite and >= 0");
+    if (!per_parcel_rho && (!std::isfinite(rho_d) || !(rho_d > 0)))
+        return fail(ST_E_INVALID, "rho_d must be finite and > 0");
+    if (!per_parcel_rho && (!std::isfinite(rho_d_by_rho_c) || rho_d_by_rho_c < 0))
+        return fail(ST_E_INVALID, "rho_d_by_rho_c must be finite and >= 0");
+    if (per_parcel_rho && (!std::isfinite(rho_c) || !(rho_c > 0)))
+        return fail(ST_E_INVALID, "rho_c must be finite and > 0 with per-parcel density");
+    st_cloud *c = new (std::nothrow) st_cloud;
+    if (!c) return fail(ST_E_NOMEM, "host allocation failed");
+    c->ctx = ctx;
+    c->prec = precision;
+    c->es = precision == ST_FP64 ? 8 : 4;
+    c->n = n;
+    c->mu_c = mu_c;
+    c->rho_d = per_parcel_rho ? 1.0 : rho_d;
+    c->rdr = per_parcel_rho ? 0.0 : rho_d_by_rho_c;
+    c->rho_c = per_parcel_rho ? rho_c : 0.0;
+    c->has_rho = per_parcel_rho ? 1 : 0;
+    int r = alloc_set(c, 0);
+    hipError_t e = hipSuccess;
+    if (r == ST_OK) e = hipMalloc((void **)&c->bbox, 6 * sizeof(unsigned long long));
+    if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->strays, 4);
+    if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->bad, 4);
+    if (r == ST_OK && e == hipSuccess) e = hipHostMalloc((void **)&c->strays_host, 4, hipHostMallocDefault);
+    if (r == ST_OK && e == hipSuccess) e = hipEventCreateWithFlags(&c->strays_ev, hipEventDisableTiming);
+    if (r == ST_OK && e == hipSuccess) e = st::launch_iota(c->pid[0], (uint32_t)n, ctx->stream);
+    if (r != ST_OK || e != hipSuccess) {
+        std::string msg = r != ST_OK ? g_err : std::string(hipGetErrorString(e));
+        st_cloud_destroy(c);
+        return fail(r != ST_OK ? r : ST_E_DEVICE, "cloud allocation: %s", msg.c_str());
+    }
+    *out = c;
+    return ST_OK;
+}
+
+int st_cloud_destroy(st_cloud *c) {
+    if (!c) return ST_OK;
+    if (c->ctx) {
+        hipSetDevice(c->ctx->device);
+        hipStreamSynchronize(c->ctx->stream);
+    }
+    for (int w = 0; w < 2; ++w) {
+        for (int f = 0; f < kNF; ++f)
+            if (c->buf[w][f]) hipFree(c->buf[w][f]);
+        if (c->pid[w]) hipFree(c->pid[w]);
+    }
+    if (c->keys) hipFree(c->keys);
+    if (c->counts) hipFree(c->counts);
+    if (c->cursor) hipFree(c->cursor);
+    if (c->brick_start) hipFree(c->brick_start);
+    if (c->bbox) hipFree(c->bbox);
+    if (c->strays) hipFree(c->strays);
+    if (c->bad) hipFree(c->bad);
+    if (c->strays_host) hipHostFree(c->strays_host);
+    if (c->strays_ev) hipEventDestroy(c->strays_ev);
+    delete c;
+    return ST_OK;
+}
+
+int st_cloud_upload(st_cloud *c, const void *d, const void *x, const void *y, const void *z,
+                    const void *u, const void *v, const void *w, const void *rho) {
+    if (!c) return fail(ST_E_INVALID, "null cloud");
+    ST_TRY(check_ctx(c->ctx));
+    const void *src[kNF] = {d, x, y, z, u, v, w, rho};
+    for (int f = 0; f < 7; ++f)
+        if (!src[f] && c->n) return fail(ST_E_INVALID, "null parcel array %d", f);
+    if (c->has_rho && !rho && c->n) return fail(ST_E_INVALID, "cloud has per-parcel density but rho is NULL");
+    for (int f = 0; f < kNF; ++f) {
+        if (f == 7 && !c->has_rho) continue;
+        if (c->n && !finite_all(src[f], c->n, c->es)) return fail(ST_E_INVALID, "parcel array %d holds a non-finite value", f);
+    }
+    if (c->n && !positive_all(d, c->n, c->es)) return fail(ST_E_INVALID, "parcel diameters must be > 0");
+    if (c->has_rho && c->n && !positive_all(rho, c->n, c->es)) return fail(ST_E_INVALID, "parcel densities must be > 0");
+    hipStream_t s = c->ctx->stream;
+    c->cur = 0;
+    const size_t bytes = (size_t)c->n * c->es;
+    for (int f = 0; f < kNF; ++f) {
+        if (f == 7 && !c->has_rho) continue;
+        if (bytes) ST_HIP(hipMemcpyAsync(c->buf[0][f], src[f], bytes, hipMemcpyHostToDevice, s));
+    }
+    ST_HIP(st::launch_iota(c->pid[0], (uint32_t)c->n, s));
+    ST_HIP(hipStreamSynchronize(s));
+    c->uploaded = true;
+    c->fresh = true;
+    c->permuted = false;
+    c->sorted = false;
+    c->strays_pending = false;
+    return ST_OK;
+}
+
+int st_cloud_download(st_cloud *c, void *d, void *x, void *y, void *z, void *u, void *v, void *w) {
+    if (!c) return fail(ST_E_INVALID, "null cloud");
+    ST_TRY(check_ctx(c->ctx));
+    void *dst[7] = {d, x, y, z, u, v, w};
+    for (int f = 0; f < 7; ++f)
+        if (!dst[f] && c->n) return fail(ST_E_INVALID, "null output array %d", f);
+    if (!c->uploaded) return fail(ST_E_STATE, "cloud has no parcels uploaded");
+    hipStream_t s = c->ctx->stream;
+    int from = c->cur;
+    if (c->permuted) {
+        ST_TRY(alloc_set(c, 1 - c->cur));
+        if (c->prec == ST_FP64)
+            ST_HIP(st::launch_unpermute<double>(soa<double>(c, c->cur), c->pid[c->cur], soa<double>(c, 1 - c->cur), (uint32_t)c->n, s));
+        else
+            ST_HIP(st::launch_unpermute<float>(soa<float>(c, c->cur), c->pid[c->cur], soa<float>(c, 1 - c->cur), (uint32_t)c->n, s));
+        from = 1 - c->cur;
+    }
+    const size_t bytes = (size_t)c->n * c->es;
+    for (int f = 0; f < 7; ++f)
+        if (bytes) ST_HIP(hipMemcpyAsync(dst[f], c->buf[from][f], bytes, hipMemcpyDeviceToHost, s));
+    ST_HIP(hipStreamSynchronize(s));
+    return ST_OK;
+}
+
+int st_cloud_count(const st_cloud *c, int64_t *n) {
+    if (!c || !n) return fail(ST_E_INVALID, "null argument");
+    *n = c->n;
+    return ST_OK;
+}
+
+int st_field_create(st_ctx *ctx, const st_mesh *mesh, int32_t precision, st_field **out) {
+    if (!out) return fail(ST_E_INVALID, "null output handle");
+    *out = nullptr;
+    ST_TRY(check_ctx(ctx));
+    if (!mesh) return fail(ST_E_INVALID, "null mesh");
+    if (precision != ST_FP64 && precision != ST_FP32) return fail(ST_E_INVALID, "precision must be ST_FP64 or ST_FP32");
+    int64_t nc = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (mesh->n[a] < 1) return fail(ST_E_INVALID, "mesh n[%d] = %d must be >= 1", a, mesh->n[a]);
+        if (!std::isfinite(mesh->origin[a]) || !std::isfinite(mesh->ending[a]) || !(mesh->ending[a] > mesh->origin[a]))
+            return fail(ST_E_INVALID, "mesh axis %d needs finite origin < ending", a);
+        nc *= mesh->n[a];
+    }
+    if (nc * 3 >= 0x7fffffffLL) return fail(ST_E_RANGE, "mesh of %lld cells is above the 32-bit index limit", (long long)nc);
+    st_field *f = new (std::nothrow) st_field;
+    if (!f) return fail(ST_E_NOMEM, "host allocation failed");
+    f->ctx = ctx;
+    f->mesh = *mesh;
+    f->geo = make_geo(*mesh);
+    f->prec = precision;
+    f->es = precision == ST_FP64 ? 8 : 4;
+    f->ncell = nc;
+    hipError_t e = hipMalloc(&f->U, (size_t)nc * 3 * f->es);
+    if (e == hipSuccess) e = hipMalloc((void **)&f->src, (size_t)nc * 3 * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&f->stray, (size_t)nc * 3 * 8);
+    if (e == hipSuccess) e = st::launch_fill_f64(f->src, 0.0, (size_t)nc * 3, ctx->stream);
+    if (e == hipSuccess) e = st::launch_fill_f64(f->stray, 0.0, (size_t)nc * 3, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        st_field_destroy(f);
+        return fail(e == hipErrorOutOfMemory ? ST_E_NOMEM : ST_E_DEVICE, "field allocation: %s", hipGetErrorString(e));
+    }
+    *out = f;
+    return ST_OK;
+}
+
+int st_field_destroy(st_field *f) {
+    if (!f) return ST_OK;
+    if (f->ctx) {
+        hipSetDevice(f->ctx->device);
+        hipStreamSynchronize(f->ctx->stream);
+    }
+    if (f->U) hipFree(f->U);
+    if (f->src) hipFree(f->src);
+    if (f->stray) hipFree(f->stray);
+    delete f;
+    return ST_OK;
+}
+
+int st_field_upload_U(st_field *f, const void *U, int32_t elem_bytes) {
+    if (!f) return fail(ST_E_INVALID, "null field");
+    ST_TRY(check_ctx(f->ctx));
+    if (!U) return fail(ST_E_INVALID, "null U");
+    if (elem_bytes != 4 && elem_bytes != 8) return fail(ST_E_INVALID, "elem_bytes must be 4 or 8 (got %d)", elem_bytes);
+    const int64_t n = f->ncell * 3;
+    if (!finite_all(U, n, (size_t)elem_bytes)) return fail(ST_E_INVALID, "U holds a non-finite value");
+    hipStream_t s = f->ctx->stream;
+    if ((size_t)elem_bytes == f->es) {
+        ST_HIP(hipMemcpyAsync(f->U, U, (size_t)n * f->es, hipMemcpyHostToDevice, s));
+        ST_HIP(hipStreamSynchronize(s));
+    } else {
+        std::vector<unsigned char> tmp((size_t)n * f->es);
+        if (f->es == 8) {
+            const float *a = (const float *)U;
+            double *b = (double *)tmp.data();
+            for (int64_t i = 0; i < n; ++i) b[i] = (double)a[i];
+        } else {
+            const double *a = (const double *)U;
+            float *b = (float *)tmp.data();
+            for (int64_t i = 0; i < n; ++i) b[i] = (float)a[i];
+        }
+        ST_HIP(hipMemcpyAsync(f->U, tmp.data(), tmp.size(), hipMemcpyHostToDevice, s));
+        ST_HIP(hipStreamSynchronize(s));
+    }
+    f->U_set = true;
+    return ST_OK;
+}
+
+int st_field_zero_source(st_field *f) {
+    if (!f) return fail(ST_E_INVALID, "null field");
+    ST_TRY(check_ctx(f->ctx));
+    ST_HIP(st::launch_fill_f64(f->src, 0.0, (size_t)f->ncell * 3, f->ctx->stream));
+    if (f->stray_dirty) ST_HIP(st::launch_fill_f64(f->stray, 0.0, (size_t)f->ncell * 3, f->ctx->stream));
+    f->stray_dirty = false;
+    return ST_OK;
+}
+
+int st_field_download_source(st_field *f, void *out, int32_t elem_bytes) {
+    if (!f) return fail(ST_E_INVALID, "null field");
+    ST_TRY(check_ctx(f->ctx));
+    if (!out) return fail(ST_E_INVALID, "null output");
+    if (elem_bytes != 4 && elem_bytes != 8) return fail(ST_E_INVALID, "elem_bytes must be 4 or 8 (got %d)", elem_bytes);
+    hipStream_t s = f->ctx->stream;
+    const size_t n = (size_t)f->ncell * 3;
+    if (f->stray_dirty) {
+        ST_HIP(st::launch_merge(f->src, f->stray, n, s));
+        f->stray_dirty = false;
+    }
+    if (elem_bytes == 8) {
+        ST_HIP(hipMemcpyAsync(out, f->src, n * 8, hipMemcpyDeviceToHost, s));
+        ST_HIP(hipStreamSynchronize(s));
+    } else {
+        std::vector<double> tmp(n);
+        ST_HIP(hipMemcpyAsync(tmp.data(), f->src, n * 8, hipMemcpyDeviceToHost, s));
+        ST_HIP(hipStreamSynchronize(s));
+        float *o = (float *)out;
+        for (size_t i = 0; i < n; ++i) o[i] = (float)tmp[i];
+    }
+    return ST_OK;
+}
+
+int st_evolve(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int32_t nsub, const st_params *params,
+              double bbox_out[6]) {
+    st_params p;
+    ST_TRY(prepare(ctx, c, f, dt_fluid, nsub, params, p));
+    if (c->n == 0) {
+        if (bbox_out)
+            for (int k = 0; k < 6; ++k) bbox_out[k] = k < 3 ? INFINITY : -INFINITY;
+        return ST_OK;
+    }
+    if (p.method == ST_METHOD_SORTED) ST_TRY(maybe_sort(c, f, p));
+    if (c->prec == ST_FP64)
+        ST_TRY(run_passes<double>(ctx, c, f, dt_fluid, nsub, p, false));
+    else
+        ST_TRY(run_passes<float>(ctx, c, f, dt_fluid, nsub, p, false));
+    if (bbox_out) {
+        unsigned long long keys[6];
+        ST_HIP(hipMemcpyAsync(keys, c->bbox, sizeof keys, hipMemcpyDeviceToHost, ctx->stream));
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+        for (int k = 0; k < 6; ++k) bbox_out[k] = st::order_key_decode(keys[k]);
+    }
+    return ST_OK;
+}
+
+int st_cloud_stray_fraction(st_cloud *c, double *fraction) {
+    if (!c || !fraction) return fail(ST_E_INVALID, "null argument");
+    ST_TRY(check_ctx(c->ctx));
+    if (c->strays_pending) {
+        ST_HIP(hipEventSynchronize(c->strays_ev));
+        c->strays_pending = false;
+        c->last_stray_fraction = c->strays_den > 0 ? (double)*c->strays_host / (double)c->strays_den : 0.0;
+    }
+    *fraction = c->last_stray_fraction;
+    return ST_OK;
+}
+
+int st_cloud_sort(st_ctx *ctx, st_cloud *c, st_field *f, const st_params *params) {
+    st_params p;
+    ST_TRY(prepare(ctx, c, f, 1.0, 1, params, p));
+    if (c->n == 0) return ST_OK;
+    return c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p);
+}
+
+int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int32_t nsub,
+                     const st_params *params, int32_t zero_source, st_graph **out) {
+    if (!out) return fail(ST_E_INVALID, "null output handle");
+    *out = nullptr;
+    st_params p;
+    ST_TRY(prepare(ctx, c, f, dt_fluid, nsub, params, p));
+    if (c->n == 0) return fail(ST_E_INVALID, "cannot capture a step of an empty cloud");
+    if (p.method == ST_METHOD_SORTED && !sort_matches(c, f, p))
+        ST_TRY(c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    st_graph *g = new (std::nothrow) st_graph;
+    if (!g) return fail(ST_E_NOMEM, "host allocation failed");
+    g->ctx = ctx;
+    g->field = f;
+    g->owner_flush = p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER;
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(ST_E_DEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    int r = ST_OK;
+    if (zero_source) {
+        e = st::launch_fill_f64(f->src, 0.0, (size_t)f->ncell * 3, s);
+        if (e == hipSuccess && g->owner_flush) e = st::launch_fill_f64(f->stray, 0.0, (size_t)f->ncell * 3, s);
+        if (e != hipSuccess) r = fail(ST_E_DEVICE, "capture fill: %s", hipGetErrorString(e));
+    }
+    if (r == ST_OK)
+        r = c->prec == ST_FP64 ? run_passes<double>(ctx, c, f, dt_fluid, nsub, p, true)
+                               : run_passes<float>(ctx, c, f, dt_fluid, nsub, p, true);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(s, &graph);
+    if (r != ST_OK || e != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        delete g;
+        return r != ST_OK ? r : fail(ST_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        delete g;
+        return fail(ST_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    g->graph = graph;
+    *out = g;
+    return ST_OK;
+}
+
+int st_graph_launch(st_graph *g) {
+    if (!g) return fail(ST_E_INVALID, "null graph");
+    ST_TRY(check_ctx(g->ctx));
+    ST_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
+    if (g->owner_flush) g->field->stray_dirty = true;
+    return ST_OK;
+}
+
+int st_graph_destroy(st_graph *g) {
+    if (!g) return ST_OK;
+    if (g->ctx) {
+        hipSetDevice(g->ctx->device);
+        hipStreamSynchronize(g->ctx->stream);
+    }
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return ST_OK;
+}
+
+int st_last_kernel_ms(st_ctx *ctx, double *ms, int32_t *launches) {
+    ST_TRY(check_ctx(ctx));
+    if (!ms || !launches) return fail(ST_E_INVALID, "null argument");
+    *ms = -1.0;
+    *launches = ctx->timed_launches;
+    if (ctx->timed_launches <= 0) return ST_OK;
+    ST_HIP(hipEventSynchronize(ctx->ev[1]));
+    float t = 0.f;
+    ST_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
+    *ms = (double)t / ctx->timed_launches;
+    return ST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// st_internal.h -- device-side data structures of the gfx950 parcel step.
+//
+// Layout in HBM (all arrays are separate allocations, 256-byte aligned by hipMalloc):
+//   parcel state  SoA: d, x, y, z, u, v, w [, rho], element T (float or double)
+//   pid           int32 original index of each device slot (un-permutes downloads)
+//   U             AoS xyz per cell, element T, cells x fastest
+//   src, stray    AoS xyz per cell, double
+//   brick_start   int32[nbricks + 1] CSR offsets of the brick-sorted parcel array
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace st {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Mesh as the kernels see it: the face test (x - o) * rd runs in double in both
+// precision forms.  rd = 1 / (L / N) is computed once on the host in double, as the
+// reference's construct_mesh does in its scalar type (asyncParallelTracking.jl:239-241).
+struct Geo {
+    int32_t n[3];
+    uint32_t nxny;
+    double o[3];
+    double rd[3];
+};
+
+// Parcel-phase constants in the state precision T.
+template <class T>
+struct Phys {
+    T dt;      // substep = (T)dt_fluid / (T)nsub
+    T mu;      // carrier viscosity
+    T rho;     // chunk parcel density
+    T cd;      // drag-coefficient multiplier
+    T L[3];    // periods (periodic boundary)
+    double rdr;    // chunk rho_d / rho_c
+    double rho_c;  // carrier density for per-parcel rho
+};
+
+template <class T>
+struct Soa {
+    T *d, *x, *y, *z, *u, *v, *w;
+    const T *rho;  // nullptr unless per-parcel density
+};
+
+// Bounding box and stray counter of one launch sequence.
+struct Stats {
+    unsigned long long *bbox;  // 6 order-preserving keys: min xyz, max xyz
+    unsigned int *strays;      // parcels found outside their brick (sorted method)
+};
+
+template <class T>
+struct PassArgs {
+    Soa<T> s;
+    const T *U;
+    double *src;    // owner sums (or everything, ST_FLUSH_ATOMIC)
+    double *stray;  // deposits outside the depositing parcel's brick (ST_FLUSH_OWNER)
+    const int32_t *brick_start;
+    Geo g;
+    Phys<T> ph;
+    Stats st;
+    int32_t nbricks;
+    int32_t nb[3];   // bricks per direction
+    int32_t bd[3];   // cells per brick per direction
+    int32_t tile_cells;  // bd[0]*bd[1]*bd[2]
+    int32_t nsteps;
+    int32_t flush;   // 0 owner, 1 atomic
+    uint32_t np;
+};
+
+// Order-preserving map of a double onto uint64 (for atomicMin/atomicMax).
+__host__ __device__ inline unsigned long long order_key(double x) {
+    union { double d; unsigned long long u; } c;
+    c.d = x;
+    return (c.u >> 63) ? ~c.u : (c.u | 0x8000000000000000ull);
+}
+__host__ __device__ inline double order_key_decode(unsigned long long k) {
+    union { double d; unsigned long long u; } c;
+    c.u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return c.d;
+}
+
+}  // namespace st
+
+// Host-side launchers (st_step.hip).
+namespace st {
+template <class T>
+hipError_t launch_pass_sorted(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
+template <class T>
+hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
+hipError_t launch_reset_stats(Stats st, hipStream_t s);
+template <class T>
+hipError_t launch_brick_keys(const Soa<T> &s, uint32_t np, Geo g, const int32_t nb[3],
+                             const int32_t bd[3], int periodic, uint32_t *keys,
+                             uint32_t *counts, unsigned int *bad, hipStream_t st);
+hipError_t launch_scan(const uint32_t *counts, int32_t nbins, int32_t *start, uint32_t *cursor,
+                       hipStream_t s);
+template <class T>
+hipError_t launch_scatter(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out,
+                          const uint32_t *keys, uint32_t *cursor, uint32_t np, hipStream_t s);
+template <class T>
+hipError_t launch_unpermute(const Soa<T> &in, const int32_t *pid, Soa<T> out, uint32_t np,
+                            hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+hipError_t launch_fill_f64(double *p, double value, size_t n, hipStream_t s);
+hipError_t launch_iota(int32_t *p, uint32_t n, hipStream_t s);
+hipError_t launch_merge(double *src, double *stray, size_t n, hipStream_t s);
+template <class T>
+hipError_t launch_range_check(const Soa<T> &s, uint32_t np, Geo g, unsigned int *bad,
+                              hipStream_t st);
+}  // namespace st

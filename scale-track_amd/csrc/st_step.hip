@@ -1,0 +1,718 @@
+// st_step.hip -- gfx950 kernels of the Lagrangian parcel step.
+//
+// Physics and event order follow evolve_particle! (run/icoJuliaParcelFoam/cavity3D/
+// asyncParallelTracking.jl:1271-1365): cell-constant carrier velocity, implicit-Euler
+// Stokes drag, relocation, specular wall bounce gated on the folded cell index
+// changing, and a momentum deposit into the cell each residence started in.  The
+// whole library is compiled with -ffp-contract=off, so each expression below rounds
+// exactly as the reference's left-to-right Julia arithmetic does.
+//
+// Two launch shapes:
+//  * k_pass_plain  -- one thread per parcel in device order, velocity gathered from
+//    HBM, three FP64 atomics per deposit (the reference's shape, :1257-1269, :710-728).
+//  * k_pass_sorted -- parcels sorted by mesh brick; one workgroup per brick stages the
+//    brick's cell velocities in LDS, sums its deposits in LDS (FP64 ds_add), and adds
+//    them to HBM once per cell.  Parcels that drifted out of their brick fall back to
+//    HBM gathers and FP64 atomics into a separate array.
+#include "st_internal.h"
+
+namespace st {
+
+constexpr int kWave = 64;
+constexpr int kXcd = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+// Workgroup id -> brick id so that each XCD (and its L2) gets one contiguous range of
+// bricks instead of every 8th brick.
+__device__ __forceinline__ int32_t xcd_remap(int32_t bid, int32_t n) {
+    const int32_t q = n / kXcd, r = n % kXcd;
+    const int32_t x = bid % kXcd, j = bid / kXcd;
+    return x * q + (x < r ? x : r) + j;
+}
+
+__device__ __forceinline__ int32_t axis_floor(double x, double o, double rd) {
+    return (int32_t)floor((x - o) * rd);
+}
+
+// divrem by the partition size (= N for one partition) with truncation toward zero,
+// then abs() of both parts (asyncParallelTracking.jl:671-696).
+__device__ __forceinline__ void fold(int32_t I, int32_t N, int32_t &f, int32_t &p) {
+    if ((uint32_t)I < (uint32_t)N) {
+        f = I;
+        p = 0;
+    } else {
+        const int32_t q = I / N;
+        const int32_t r = I - q * N;
+        f = r < 0 ? -r : r;
+        p = q < 0 ? -q : q;
+    }
+}
+
+__device__ __forceinline__ int32_t clampi(int32_t i, int32_t n) {
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+struct Cell {
+    int32_t f[3];  // folded per-axis indices, always in [0, N)
+    uint32_t c;    // linear cell, x fastest
+    int32_t part;  // folded partition index (wall mode)
+};
+
+__device__ __forceinline__ void cell_wall(const Geo &g, const int32_t I[3], Cell &cl) {
+    int32_t p0, p1, p2;
+    fold(I[0], g.n[0], cl.f[0], p0);
+    fold(I[1], g.n[1], cl.f[1], p1);
+    fold(I[2], g.n[2], cl.f[2], p2);
+    cl.part = p0 + p1 + p2;
+    cl.c = (uint32_t)cl.f[2] * g.nxny + (uint32_t)cl.f[1] * (uint32_t)g.n[0] + (uint32_t)cl.f[0];
+}
+
+__device__ __forceinline__ void cell_clamped(const Geo &g, const int32_t I[3], Cell &cl) {
+    cl.f[0] = clampi(I[0], g.n[0]);
+    cl.f[1] = clampi(I[1], g.n[1]);
+    cl.f[2] = clampi(I[2], g.n[2]);
+    cl.part = 0;
+    cl.c = (uint32_t)cl.f[2] * g.nxny + (uint32_t)cl.f[1] * (uint32_t)g.n[0] + (uint32_t)cl.f[0];
+}
+
+template <class T, bool PERIODIC>
+__device__ __forceinline__ void locate(const Geo &g, T x, T y, T z, int32_t I[3], Cell &cl) {
+    I[0] = axis_floor((double)x, g.o[0], g.rd[0]);
+    I[1] = axis_floor((double)y, g.o[1], g.rd[1]);
+    I[2] = axis_floor((double)z, g.o[2], g.rd[2]);
+    if (PERIODIC)
+        cell_clamped(g, I, cl);
+    else
+        cell_wall(g, I, cl);
+}
+
+__device__ __forceinline__ void atomic_add_global(double *p, double v) {
+    unsafeAtomicAdd(p, v);  // global_atomic_add_f64 (coarse-grained hipMalloc memory)
+}
+__device__ __forceinline__ void atomic_add_lds(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Everything in HBM: the plain method.
+template <class T>
+struct GlobalEnv {
+    const T *__restrict__ U;
+    double *__restrict__ src;
+    __device__ __forceinline__ void enter(const Cell &) {}
+    __device__ __forceinline__ void load_u(const Cell &cl, T &ux, T &uy, T &uz) const {
+        const T *q = U + 3u * cl.c;
+        ux = q[0];
+        uy = q[1];
+        uz = q[2];
+    }
+    __device__ __forceinline__ void deposit(const Cell &cl, double a, double b, double c) {
+        double *q = src + 3u * cl.c;
+        atomic_add_global(q, a);
+        atomic_add_global(q + 1, b);
+        atomic_add_global(q + 2, c);
+    }
+};
+
+// A brick staged in LDS; cells outside it fall back to HBM.
+template <class T>
+struct TileEnv {
+    const T *__restrict__ U;
+    double *__restrict__ out;  // stray array (owner flush) or the source (atomic flush)
+    const T *Ut;               // LDS velocity tile, AoS
+    double *St;                // LDS source tile, AoS
+    int32_t o0, o1, o2;        // tile origin (cells)
+    uint32_t e0, e1, e2;       // tile extent (cells)
+    uint32_t strays;
+    __device__ __forceinline__ bool local(const Cell &cl, uint32_t &l) const {
+        const uint32_t a = (uint32_t)(cl.f[0] - o0);
+        const uint32_t b = (uint32_t)(cl.f[1] - o1);
+        const uint32_t c = (uint32_t)(cl.f[2] - o2);
+        l = a + e0 * (b + e1 * c);
+        return a < e0 && b < e1 && c < e2;
+    }
+    __device__ __forceinline__ void enter(const Cell &cl) {
+        uint32_t l;
+        if (!local(cl, l)) ++strays;
+    }
+    __device__ __forceinline__ void load_u(const Cell &cl, T &ux, T &uy, T &uz) const {
+        uint32_t l;
+        // Two explicit paths: a select between an LDS and a global pointer would
+        // make the compiler spill both into a private array.
+        if (local(cl, l)) {
+            ux = Ut[3u * l];
+            uy = Ut[3u * l + 1];
+            uz = Ut[3u * l + 2];
+        } else {
+            const T *q = U + 3u * cl.c;
+            ux = q[0];
+            uy = q[1];
+            uz = q[2];
+        }
+    }
+    __device__ __forceinline__ void deposit(const Cell &cl, double a, double b, double c) {
+        uint32_t l;
+        if (local(cl, l)) {
+            double *q = St + 3u * l;
+            atomic_add_lds(q, a);
+            atomic_add_lds(q + 1, b);
+            atomic_add_lds(q + 2, c);
+        } else {
+            double *q = out + 3u * cl.c;
+            atomic_add_global(q, a);
+            atomic_add_global(q + 1, b);
+            atomic_add_global(q + 2, c);
+        }
+    }
+};
+
+template <class T>
+struct Box {
+    T lo[3], hi[3];
+    __device__ __forceinline__ void init() {
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = (T)INFINITY;
+            hi[c] = -(T)INFINITY;
+        }
+    }
+    __device__ __forceinline__ void add(T x, T y, T z) {
+        lo[0] = fmin(lo[0], x);
+        lo[1] = fmin(lo[1], y);
+        lo[2] = fmin(lo[2], z);
+        hi[0] = fmax(hi[0], x);
+        hi[1] = fmax(hi[1], y);
+        hi[2] = fmax(hi[2], z);
+    }
+};
+
+// One parcel, nsteps substeps (asyncParallelTracking.jl:1271-1365).
+template <class T, bool PERIODIC, class Env>
+__device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
+                                            uint32_t p, int32_t nsteps, Env &env, Box<T> &box) {
+    const T dia = s.d[p];
+    T rho = ph.rho;
+    double rdr = ph.rdr;
+    if (s.rho) {
+        rho = s.rho[p];
+        rdr = (double)rho / ph.rho_c;
+    }
+    // :1280 mass over carrier density, in double in both precision forms
+    const double dd = (double)dia;
+    const double m = ((rdr * kPi) * ((dd * dd) * dd)) / 6.0;
+
+    T px = s.x[p], py = s.y[p], pz = s.z[p];
+    T vx = s.u[p], vy = s.v[p], vz = s.w[p];
+
+    int32_t I[3];
+    Cell cell;
+    locate<T, PERIODIC>(g, px, py, pz, I, cell);  // :1286
+    const int32_t part0 = cell.part;
+    env.enter(cell);
+    T ux, uy, uz;
+    env.load_u(cell, ux, uy, uz);  // :1287-1288
+    double s0x = (double)vx, s0y = (double)vy, s0z = (double)vz;  // :1289
+
+    // :1294 -- the drag factor depends on neither position nor velocity, so it is the
+    // same value in every substep.
+    T f = (((T)18 * ph.mu) * ph.dt) / (rho * (dia * dia));
+    f = ph.cd * f;
+    const T den = (T)1 + f;
+
+    for (int32_t t = 1; t <= nsteps; ++t) {
+        // :1295-1296
+        T nvx = (vx + f * ux) / den;
+        T nvy = (vy + f * uy) / den;
+        T nvz = (vz + f * uz) / den;
+        T nx = px + nvx * ph.dt;
+        T ny = py + nvy * ph.dt;
+        T nz = pz + nvz * ph.dt;
+        Cell nc;
+        if (!PERIODIC) {
+            locate<T, false>(g, nx, ny, nz, I, nc);  // :1303
+            if (nc.c != cell.c || nc.part != part0) {  // :1306
+                bool hit = false;
+                if (I[0] < 0 || I[0] >= g.n[0]) {  // :1309-1329
+                    nx = px;
+                    nvx = -vx;
+                    s0x = s0x - 2.0 * (double)vx;
+                    hit = true;
+                }
+                if (I[1] < 0 || I[1] >= g.n[1]) {
+                    ny = py;
+                    nvy = -vy;
+                    s0y = s0y - 2.0 * (double)vy;
+                    hit = true;
+                }
+                if (I[2] < 0 || I[2] >= g.n[2]) {
+                    nz = pz;
+                    nvz = -vz;
+                    s0z = s0z - 2.0 * (double)vz;
+                    hit = true;
+                }
+                if (hit) locate<T, false>(g, nx, ny, nz, I, nc);  // :1333-1335
+                if (t != nsteps && nc.c != cell.c) {                // :1342-1348
+                    env.deposit(cell, m * (s0x - (double)nvx), m * (s0y - (double)nvy),
+                                m * (s0z - (double)nvz));
+                    s0x = (double)nvx;
+                    s0y = (double)nvy;
+                    s0z = (double)nvz;
+                    cell = nc;
+                    env.load_u(cell, ux, uy, uz);
+                }
+            }
+        } else {
+            // periodic extension: wrap each axis that left [0, N) by one period
+            I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
+            I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
+            I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
+            if (I[0] < 0) {
+                nx = nx + ph.L[0];
+                I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
+            } else if (I[0] >= g.n[0]) {
+                nx = nx - ph.L[0];
+                I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
+            }
+            if (I[1] < 0) {
+                ny = ny + ph.L[1];
+                I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
+            } else if (I[1] >= g.n[1]) {
+                ny = ny - ph.L[1];
+                I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
+            }
+            if (I[2] < 0) {
+                nz = nz + ph.L[2];
+                I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
+            } else if (I[2] >= g.n[2]) {
+                nz = nz - ph.L[2];
+                I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
+            }
+            cell_clamped(g, I, nc);
+            if (t != nsteps && nc.c != cell.c) {
+                env.deposit(cell, m * (s0x - (double)nvx), m * (s0y - (double)nvy),
+                            m * (s0z - (double)nvz));
+                s0x = (double)nvx;
+                s0y = (double)nvy;
+                s0z = (double)nvz;
+                cell = nc;
+                env.load_u(cell, ux, uy, uz);
+            }
+        }
+        vx = nvx;  // :1351-1352
+        vy = nvy;
+        vz = nvz;
+        px = nx;
+        py = ny;
+        pz = nz;
+    }
+    // :1355-1361
+    s.x[p] = px;
+    s.y[p] = py;
+    s.z[p] = pz;
+    s.u[p] = vx;
+    s.v[p] = vy;
+    s.w[p] = vz;
+    env.deposit(cell, m * (s0x - (double)vx), m * (s0y - (double)vy), m * (s0z - (double)vz));
+    box.add(px, py, pz);  // :1362
+}
+
+// Workgroup reduction of the box and the stray count, then one atomic per value.
+// Every thread of the workgroup must call it (it contains barriers).
+template <class T>
+__device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, const Stats &st) {
+    __shared__ double red[6][1024 / kWave];
+    __shared__ uint32_t redc[1024 / kWave];
+    double v[6];
+    for (int c = 0; c < 3; ++c) {
+        v[c] = (double)box.lo[c];
+        v[3 + c] = (double)box.hi[c];
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        for (int c = 0; c < 3; ++c) {
+            v[c] = fmin(v[c], __shfl_xor(v[c], off, kWave));
+            v[3 + c] = fmax(v[3 + c], __shfl_xor(v[3 + c], off, kWave));
+        }
+        strays += __shfl_xor(strays, off, kWave);
+    }
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+    if (lane == 0) {
+        for (int c = 0; c < 6; ++c) red[c][wid] = v[c];
+        redc[wid] = strays;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nw; ++w) {
+            for (int c = 0; c < 3; ++c) {
+                v[c] = fmin(v[c], red[c][w]);
+                v[3 + c] = fmax(v[3 + c], red[3 + c][w]);
+            }
+            strays += redc[w];
+        }
+        for (int c = 0; c < 3; ++c) {
+            if (v[c] <= v[3 + c]) {  // skip workgroups without parcels
+                atomicMin(st.bbox + c, order_key(v[c]));
+                atomicMax(st.bbox + 3 + c, order_key(v[3 + c]));
+            }
+        }
+        if (strays) atomicAdd(st.strays, strays);
+    }
+}
+
+template <class T, bool PERIODIC>
+__global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    Box<T> box;
+    box.init();
+    GlobalEnv<T> env{a.U, a.src};
+    if (p < a.np) step_parcel<T, PERIODIC>(a.g, a.ph, a.s, p, a.nsteps, env, box);
+    flush_stats(box, 0u, a.st);
+}
+
+template <class T, bool PERIODIC>
+__global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
+    extern __shared__ double lds[];
+    const int32_t brick = xcd_remap(blockIdx.x, a.nbricks);
+    const int32_t bi = brick % a.nb[0];
+    const int32_t bj = (brick / a.nb[0]) % a.nb[1];
+    const int32_t bk = brick / (a.nb[0] * a.nb[1]);
+
+    TileEnv<T> env;
+    env.U = a.U;
+    env.out = a.flush == 0 ? a.stray : a.src;
+    env.o0 = bi * a.bd[0];
+    env.o1 = bj * a.bd[1];
+    env.o2 = bk * a.bd[2];
+    const int32_t ex = min(a.bd[0], a.g.n[0] - env.o0);
+    const int32_t ey = min(a.bd[1], a.g.n[1] - env.o1);
+    const int32_t ez = min(a.bd[2], a.g.n[2] - env.o2);
+    env.e0 = (uint32_t)ex;
+    env.e1 = (uint32_t)ey;
+    env.e2 = (uint32_t)ez;
+    env.strays = 0;
+    const int32_t ncell = ex * ey * ez;
+    double *St = lds;
+    T *Ut = (T *)(lds + 3 * a.tile_cells);
+    env.St = St;
+    env.Ut = Ut;
+
+    // Stage the brick's velocities (rows of e[0] contiguous cells) and clear its sums.
+    for (int32_t t = threadIdx.x; t < ncell; t += blockDim.x) {
+        const int32_t li = t % ex;
+        const int32_t r = t / ex;
+        const int32_t lj = r % ey;
+        const int32_t lk = r / ey;
+        const uint32_t c = (uint32_t)(env.o2 + lk) * a.g.nxny +
+                           (uint32_t)(env.o1 + lj) * (uint32_t)a.g.n[0] + (uint32_t)(env.o0 + li);
+        const T *q = a.U + 3u * c;
+        Ut[3 * t] = q[0];
+        Ut[3 * t + 1] = q[1];
+        Ut[3 * t + 2] = q[2];
+        St[3 * t] = 0.0;
+        St[3 * t + 1] = 0.0;
+        St[3 * t + 2] = 0.0;
+    }
+    __syncthreads();
+
+    Box<T> box;
+    box.init();
+    const int32_t p0 = a.brick_start[brick], p1 = a.brick_start[brick + 1];
+    for (int32_t p = p0 + (int32_t)threadIdx.x; p < p1; p += blockDim.x)
+        step_parcel<T, PERIODIC>(a.g, a.ph, a.s, (uint32_t)p, a.nsteps, env, box);
+    __syncthreads();
+
+    // Add the brick's sums to HBM once per cell.  With the owner flush only this
+    // workgroup touches these cells of `src` during the launch (strays went to
+    // `stray`), so a plain read-modify-write is race-free.
+    for (int32_t t = threadIdx.x; t < ncell; t += blockDim.x) {
+        const double a0 = St[3 * t], a1 = St[3 * t + 1], a2 = St[3 * t + 2];
+        if (a0 == 0.0 && a1 == 0.0 && a2 == 0.0) continue;
+        const int32_t li = t % ex;
+        const int32_t r = t / ex;
+        const int32_t lj = r % ey;
+        const int32_t lk = r / ey;
+        const uint32_t c = (uint32_t)(env.o2 + lk) * a.g.nxny +
+                           (uint32_t)(env.o1 + lj) * (uint32_t)a.g.n[0] + (uint32_t)(env.o0 + li);
+        double *q = a.src + 3u * c;
+        if (a.flush == 0) {
+            q[0] += a0;
+            q[1] += a1;
+            q[2] += a2;
+        } else {
+            atomic_add_global(q, a0);
+            atomic_add_global(q + 1, a1);
+            atomic_add_global(q + 2, a2);
+        }
+    }
+    flush_stats(box, env.strays, a.st);
+}
+
+template <class T>
+hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s) {
+    if (a.np == 0) return hipSuccess;
+    const uint32_t grid = (a.np + block - 1) / block;
+    if (periodic)
+        k_pass_plain<T, true><<<grid, block, 0, s>>>(a);
+    else
+        k_pass_plain<T, false><<<grid, block, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_pass_sorted(const PassArgs<T> &a, int periodic, int block, hipStream_t s) {
+    const size_t lds = (size_t)a.tile_cells * 3 * (sizeof(double) + sizeof(T));
+    if (periodic)
+        k_pass_sorted<T, true><<<a.nbricks, block, lds, s>>>(a);
+    else
+        k_pass_sorted<T, false><<<a.nbricks, block, lds, s>>>(a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_pass_plain<float>(const PassArgs<float> &, int, int, hipStream_t);
+template hipError_t launch_pass_plain<double>(const PassArgs<double> &, int, int, hipStream_t);
+template hipError_t launch_pass_sorted<float>(const PassArgs<float> &, int, int, hipStream_t);
+template hipError_t launch_pass_sorted<double>(const PassArgs<double> &, int, int, hipStream_t);
+
+// ---------------------------------------------------------------------------------
+// Small utility kernels (graph-capturable: no hipMemsetAsync anywhere).
+
+__global__ void k_reset_stats(Stats st) {
+    const int t = threadIdx.x;
+    if (t < 3) st.bbox[t] = order_key(INFINITY);
+    else if (t < 6) st.bbox[t] = order_key(-INFINITY);
+    else if (t == 6) st.strays[0] = 0u;
+}
+
+hipError_t launch_reset_stats(Stats st, hipStream_t s) {
+    k_reset_stats<<<1, 64, 0, s>>>(st);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_f64(double *p, double v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_iota(int32_t *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+__global__ void k_merge(double *src, double *stray, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double s = stray[i];
+        if (s != 0.0) {
+            src[i] += s;
+            stray[i] = 0.0;
+        }
+    }
+}
+
+static unsigned grid_for(size_t n, unsigned block) {
+    size_t g = (n + block - 1) / block;
+    if (g > 65536) g = 65536;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_fill_u32<<<grid_for(n, 256), 256, 0, s>>>(p, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_fill_f64<<<grid_for(n, 256), 256, 0, s>>>(p, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_iota(int32_t *p, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_iota<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+hipError_t launch_merge(double *src, double *stray, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_merge<<<grid_for(n, 256), 256, 0, s>>>(src, stray, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Counting sort of parcels by brick.
+
+// Wave-aggregated atomicAdd: lanes with equal keys share one atomic; each active lane
+// gets base + its rank among them.  All lanes of the wave must call it.
+__device__ __forceinline__ uint32_t wave_claim(uint32_t *ctr, uint32_t key, bool active) {
+    const int lane = threadIdx.x % kWave;
+    unsigned long long pending = __ballot(active);
+    uint32_t result = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t lkey = __shfl(key, leader, kWave);
+        const bool mine = ((pending >> lane) & 1ull) && key == lkey;
+        const unsigned long long grp = __ballot(mine);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(ctr + lkey, (uint32_t)__popcll(grp));
+        base = __shfl(base, leader, kWave);
+        if (mine) result = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+        pending &= ~grp;
+    }
+    return result;
+}
+
+template <class T, bool PERIODIC>
+__global__ void k_brick_keys(Soa<T> s, uint32_t np, Geo g, int32_t nb0, int32_t nb1, int32_t bd0,
+                             int32_t bd1, int32_t bd2, uint32_t *keys, uint32_t *counts,
+                             unsigned int *bad) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = p < np;
+    uint32_t key = 0;
+    if (active) {
+        int32_t I[3];
+        Cell cl;
+        locate<T, PERIODIC>(g, s.x[p], s.y[p], s.z[p], I, cl);
+        if (!PERIODIC && cl.part != 0) atomicAdd(bad, 1u);
+        key = (uint32_t)(cl.f[0] / bd0) +
+              (uint32_t)nb0 * ((uint32_t)(cl.f[1] / bd1) + (uint32_t)nb1 * (uint32_t)(cl.f[2] / bd2));
+        keys[p] = key;
+    }
+    wave_claim(counts, key, active);
+}
+
+template <class T>
+hipError_t launch_brick_keys(const Soa<T> &s, uint32_t np, Geo g, const int32_t nb[3],
+                             const int32_t bd[3], int periodic, uint32_t *keys, uint32_t *counts,
+                             unsigned int *bad, hipStream_t st) {
+    if (!np) return hipSuccess;
+    const uint32_t grid = (np + 255) / 256;
+    if (periodic)
+        k_brick_keys<T, true><<<grid, 256, 0, st>>>(s, np, g, nb[0], nb[1], bd[0], bd[1], bd[2], keys,
+                                                     counts, bad);
+    else
+        k_brick_keys<T, false><<<grid, 256, 0, st>>>(s, np, g, nb[0], nb[1], bd[0], bd[1], bd[2], keys,
+                                                      counts, bad);
+    return hipGetLastError();
+}
+template hipError_t launch_brick_keys<float>(const Soa<float> &, uint32_t, Geo, const int32_t *,
+                                             const int32_t *, int, uint32_t *, uint32_t *,
+                                             unsigned int *, hipStream_t);
+template hipError_t launch_brick_keys<double>(const Soa<double> &, uint32_t, Geo, const int32_t *,
+                                              const int32_t *, int, uint32_t *, uint32_t *,
+                                              unsigned int *, hipStream_t);
+
+// Exclusive scan of the bin counts in one workgroup of 1024 threads: each thread
+// sums a contiguous segment, the 1024 partial sums are scanned in LDS, then each
+// thread writes its segment.  start has nbins + 1 entries; cursor = start[0..nbins).
+__global__ void __launch_bounds__(1024) k_scan(const uint32_t *counts, int32_t nbins, int32_t *start,
+                                               uint32_t *cursor) {
+    __shared__ uint32_t part[1024];
+    const int32_t t = threadIdx.x;
+    const int32_t seg = (nbins + 1023) / 1024;
+    const int32_t b0 = min(t * seg, nbins), b1 = min(b0 + seg, nbins);
+    uint32_t sum = 0;
+    for (int32_t b = b0; b < b1; ++b) sum += counts[b];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t add = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;  // exclusive prefix of this segment
+    for (int32_t b = b0; b < b1; ++b) {
+        start[b] = (int32_t)run;
+        cursor[b] = run;
+        run += counts[b];
+    }
+    if (t == 1023) start[nbins] = (int32_t)part[1023];
+}
+
+hipError_t launch_scan(const uint32_t *counts, int32_t nbins, int32_t *start, uint32_t *cursor,
+                       hipStream_t s) {
+    k_scan<<<1, 1024, 0, s>>>(counts, nbins, start, cursor);
+    return hipGetLastError();
+}
+
+template <class T>
+__global__ void k_scatter(Soa<T> in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out,
+                          const uint32_t *keys, uint32_t *cursor, uint32_t np) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = p < np;
+    const uint32_t key = active ? keys[p] : 0u;
+    const uint32_t q = wave_claim(cursor, key, active);
+    if (!active) return;
+    out.d[q] = in.d[p];
+    out.x[q] = in.x[p];
+    out.y[q] = in.y[p];
+    out.z[q] = in.z[p];
+    out.u[q] = in.u[p];
+    out.v[q] = in.v[p];
+    out.w[q] = in.w[p];
+    if (in.rho) ((T *)out.rho)[q] = in.rho[p];
+    pid_out[q] = pid_in[p];
+}
+
+template <class T>
+hipError_t launch_scatter(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out,
+                          const uint32_t *keys, uint32_t *cursor, uint32_t np, hipStream_t s) {
+    if (!np) return hipSuccess;
+    k_scatter<T><<<(np + 255) / 256, 256, 0, s>>>(in, pid_in, out, pid_out, keys, cursor, np);
+    return hipGetLastError();
+}
+template hipError_t launch_scatter<float>(const Soa<float> &, const int32_t *, Soa<float>, int32_t *,
+                                          const uint32_t *, uint32_t *, uint32_t, hipStream_t);
+template hipError_t launch_scatter<double>(const Soa<double> &, const int32_t *, Soa<double>,
+                                           int32_t *, const uint32_t *, uint32_t *, uint32_t,
+                                           hipStream_t);
+
+template <class T>
+__global__ void k_unpermute(Soa<T> in, const int32_t *pid, Soa<T> out, uint32_t np) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const uint32_t q = (uint32_t)pid[p];
+    out.d[q] = in.d[p];
+    out.x[q] = in.x[p];
+    out.y[q] = in.y[p];
+    out.z[q] = in.z[p];
+    out.u[q] = in.u[p];
+    out.v[q] = in.v[p];
+    out.w[q] = in.w[p];
+}
+
+template <class T>
+hipError_t launch_unpermute(const Soa<T> &in, const int32_t *pid, Soa<T> out, uint32_t np,
+                            hipStream_t s) {
+    if (!np) return hipSuccess;
+    k_unpermute<T><<<(np + 255) / 256, 256, 0, s>>>(in, pid, out, np);
+    return hipGetLastError();
+}
+template hipError_t launch_unpermute<float>(const Soa<float> &, const int32_t *, Soa<float>, uint32_t,
+                                            hipStream_t);
+template hipError_t launch_unpermute<double>(const Soa<double> &, const int32_t *, Soa<double>,
+                                             uint32_t, hipStream_t);
+
+// Count parcels whose position is outside [origin, ending) on some axis.
+template <class T>
+__global__ void k_range_check(Soa<T> s, uint32_t np, Geo g, unsigned int *bad) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int32_t i = axis_floor((double)s.x[p], g.o[0], g.rd[0]);
+    const int32_t j = axis_floor((double)s.y[p], g.o[1], g.rd[1]);
+    const int32_t k = axis_floor((double)s.z[p], g.o[2], g.rd[2]);
+    if ((uint32_t)i >= (uint32_t)g.n[0] || (uint32_t)j >= (uint32_t)g.n[1] ||
+        (uint32_t)k >= (uint32_t)g.n[2])
+        atomicAdd(bad, 1u);
+}
+
+template <class T>
+hipError_t launch_range_check(const Soa<T> &s, uint32_t np, Geo g, unsigned int *bad,
+                              hipStream_t st) {
+    if (!np) return hipSuccess;
+    k_range_check<T><<<(np + 255) / 256, 256, 0, st>>>(s, np, g, bad);
+    return hipGetLastError();
+}
+template hipError_t launch_range_check<float>(const Soa<float> &, uint32_t, Geo, unsigned int *,
+                                              hipStream_t);
+template hipError_t launch_range_check<double>(const Soa<double> &, uint32_t, Geo, unsigned int *,
+                                               hipStream_t);
+
+}  // namespace st

@@ -1,0 +1,198 @@
+"""Seeded synthetic inputs of the parcel step, generated once on the CPU in FP64.
+
+The same arrays are handed to the device and to any CPU check, so no transcendental
+function is ever evaluated twice on different hardware.  The generator is SplitMix64
+(Steele, Lea, Flood, "Fast splittable pseudorandom number generators", OOPSLA 2014)
+in its counter form: output i of a stream with state s is mix(s + (i + 1) * GAMMA).
+Julia's default RNG stream for the reference's seed 19891
+(run/icoJuliaParcelFoam/cavity3D/asyncParallelTracking.jl:504-531) cannot be
+reproduced without Julia, so the seed is kept and the generator is this one.
+"""
+import math
+
+import numpy as np
+
+GAMMA = np.uint64(0x9E3779B97F4A7C15)
+M1 = np.uint64(0xBF58476D1CE4E5B9)
+M2 = np.uint64(0x94D049BB133111EB)
+SEED = 19891  # the reference's seed (asyncParallelTracking.jl:514)
+
+
+def splitmix64(state, n, offset=0):
+    """n outputs of SplitMix64 started at `state`, skipping the first `offset`."""
+    with np.errstate(over="ignore"):
+        i = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(state) + i * GAMMA
+        z = (z ^ (z >> np.uint64(30))) * M1
+        z = (z ^ (z >> np.uint64(27))) * M2
+        return z ^ (z >> np.uint64(31))
+
+
+def stream_state(seed, stream):
+    """Initial state of an independent stream: mix of (seed, stream)."""
+    return int(splitmix64(np.uint64(seed) ^ np.uint64(stream * 0x632BE59BD9B4E019 % 2**64), 1)[0])
+
+
+def uniform(seed, stream, n):
+    """Uniform doubles in [0, 1) with 53 random bits."""
+    z = splitmix64(stream_state(seed, stream), n)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed, stream, n):
+    """Standard normals by Box-Muller from two uniform streams."""
+    u1 = 1.0 - uniform(seed, stream, n)  # (0, 1]
+    u2 = uniform(seed, stream + 1, n)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def positions(seed, n, origin, ending, dtype=np.float64, stream=10):
+    """Uniform positions in [origin, ending) per axis; after rounding to dtype every
+    coordinate is still strictly below `ending`."""
+    out = []
+    for a in range(3):
+        x = origin[a] + uniform(seed, stream + a, n) * (ending[a] - origin[a])
+        x = x.astype(dtype)
+        hi = np.asarray(ending[a], dtype=dtype)
+        bad = x >= hi
+        if bad.any():
+            x[bad] = np.nextafter(hi, np.asarray(-np.inf, dtype=dtype))
+        lo = np.asarray(origin[a], dtype=dtype)
+        x = np.maximum(x, lo)
+        out.append(x)
+    return out
+
+
+def lognormal_diameters(seed, n, median=30e-6, sigma=0.5, dtype=np.float64, stream=20):
+    return (median * np.exp(sigma * normal(seed, stream, n))).astype(dtype)
+
+
+def cell_centres(n, origin, ending):
+    """Cell-centre coordinates, each of shape (ncell,), cells numbered x fastest."""
+    axes = []
+    for a in range(3):
+        d = (ending[a] - origin[a]) / n[a]
+        axes.append(origin[a] + (np.arange(n[a]) + 0.5) * d)
+    zc, yc, xc = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return xc.ravel(), yc.ravel(), zc.ravel()
+
+
+def taylor_green(n, origin, ending, u0=1.0, phase=(0.0, 0.0, 0.0), dtype=np.float64):
+    """Frozen Taylor-Green velocity at cell centres, AoS (ncell, 3):
+    u = U0 sin(kx) cos(ky) cos(kz), v = -U0 cos(kx) sin(ky) cos(kz), w = 0,
+    k = 2 pi / L per axis."""
+    x, y, z = cell_centres(n, origin, ending)
+    kx = 2 * np.pi / (ending[0] - origin[0])
+    ky = 2 * np.pi / (ending[1] - origin[1])
+    kz = 2 * np.pi / (ending[2] - origin[2])
+    ax, ay, az = kx * x + phase[0], ky * y + phase[1], kz * z + phase[2]
+    U = np.empty((x.size, 3), dtype=np.float64)
+    U[:, 0] = u0 * np.sin(ax) * np.cos(ay) * np.cos(az)
+    U[:, 1] = -u0 * np.cos(ax) * np.sin(ay) * np.cos(az)
+    U[:, 2] = 0.0
+    return U.astype(dtype)
+
+
+def fourier_field(n, origin, ending, modes=8, seed=SEED, amplitude=1.0, kmax=4, dtype=np.float64,
+                  stream=40):
+    """Divergence-free random field from `modes` Fourier modes Re(A_m exp(i k_m . x))
+    with A_m perpendicular to k_m, evaluated at cell centres; AoS (ncell, 3)."""
+    x, y, z = cell_centres(n, origin, ending)
+    L = [ending[a] - origin[a] for a in range(3)]
+    U = np.zeros((x.size, 3))
+    r = uniform(seed, stream, modes * 9)
+    for m in range(modes):
+        q = r[9 * m: 9 * m + 9]
+        kint = np.floor(q[0:3] * (2 * kmax + 1)) - kmax
+        if not kint.any():
+            kint[0] = 1.0
+        k = 2 * np.pi * kint / np.asarray(L)
+        a = q[3:6] - 0.5
+        a = a - k * (a @ k) / (k @ k)  # project out the k component
+        na = np.linalg.norm(a)
+        if na == 0:
+            continue
+        a = a / na * amplitude / math.sqrt(modes)
+        ph = 2 * np.pi * q[6]
+        arg = k[0] * x + k[1] * y + k[2] * z + ph
+        U += np.cos(arg)[:, None] * a[None, :]
+    return U.astype(dtype)
+
+
+def cavity_field(n, origin, ending, lid=1.0, dtype=np.float64):
+    """A frozen lid-driven-cavity-like velocity for the 2D cavity case: the
+    divergence-free field of the stream function psi = lid * Ly * f(sx) * g(sy), with
+    sx, sy the coordinates scaled to [0, 1], f = sin^2(pi sx), g = sy^2 (sy - 1);
+    u = dpsi/dy, v = -dpsi/dx, w = 0.  It vanishes on the walls and u = lid * f(sx)
+    under the lid."""
+    x, y, _ = cell_centres(n, origin, ending)
+    Lx = ending[0] - origin[0]
+    Ly = ending[1] - origin[1]
+    sx = (x - origin[0]) / Lx
+    sy = (y - origin[1]) / Ly
+    f = np.sin(np.pi * sx) ** 2
+    df = 2 * np.pi * np.sin(np.pi * sx) * np.cos(np.pi * sx)
+    g = sy ** 2 * (sy - 1.0)
+    dg = 3 * sy ** 2 - 2 * sy
+    U = np.zeros((x.size, 3))
+    U[:, 0] = lid * f * dg
+    U[:, 1] = -lid * df * g * (Ly / Lx)
+    return U.astype(dtype)
+
+
+class Case:
+    """One synthetic workload: mesh, frozen carrier velocity and a parcel cloud."""
+
+    def __init__(self, n, origin, ending, U, d, x, y, z, u, v, w, rho=None, mu_c=1e-3,
+                 rho_d=1000.0, rho_c=1000.0, dt_fluid=1e-3, nsub=10, boundary=0, name=""):
+        self.n, self.origin, self.ending = tuple(n), tuple(origin), tuple(ending)
+        self.U = U
+        self.d, self.x, self.y, self.z, self.u, self.v, self.w = d, x, y, z, u, v, w
+        self.rho = rho
+        self.mu_c, self.rho_d, self.rho_c = mu_c, rho_d, rho_c
+        self.rho_d_by_rho_c = rho_d / rho_c
+        self.dt_fluid, self.nsub, self.boundary = dt_fluid, nsub, boundary
+        self.name = name
+
+    @property
+    def np(self):
+        return self.d.size
+
+    @property
+    def ncell(self):
+        return self.n[0] * self.n[1] * self.n[2]
+
+    def state(self):
+        return [a.copy() for a in (self.d, self.x, self.y, self.z, self.u, self.v, self.w)]
+
+
+def taylor_green_box(np_parcels=1 << 24, ncell=128, dtype=np.float64, seed=SEED, boundary=1,
+                     dt_fluid=1e-3, nsub=10, name="taylor-green box"):
+    """Periodic box [0,1]^3 of ncell^3 cells with frozen Taylor-Green flow (U0 = 1 m/s)
+    and parcels at uniform positions, log-normal diameters (median 30 um, sigma 0.5),
+    rho_p = 1000 kg/m^3, zero initial velocity; carrier mu = 1e-3, rho = 1e3 (the
+    reference defaults of setup.jl:12-13)."""
+    n = (ncell, ncell, ncell)
+    o, e = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    U = taylor_green(n, o, e, dtype=dtype)
+    x, y, z = positions(seed, np_parcels, o, e, dtype=dtype)
+    d = lognormal_diameters(seed, np_parcels, dtype=dtype)
+    zero = np.zeros(np_parcels, dtype=dtype)
+    return Case(n, o, e, U, d, x, y, z, zero, zero.copy(), zero.copy(), mu_c=1e-3, rho_d=1000.0,
+                rho_c=1000.0, dt_fluid=dt_fluid, nsub=nsub, boundary=boundary, name=name)
+
+
+def cavity_case(np_parcels=10000, dtype=np.float64, seed=SEED, name="2D lid-driven cavity"):
+    """20x20x1 cells on a 0.1 m square (0.01 m deep), lid 1 m/s, nu = 0.01 m^2/s;
+    parcels d = 50 um, rho = 2500 kg/m^3 at uniform positions, zero velocity; walls;
+    dt = 0.005 s.  The carrier field is frozen (no fluid solver here)."""
+    n = (20, 20, 1)
+    o, e = (0.0, 0.0, 0.0), (0.1, 0.1, 0.01)
+    U = cavity_field(n, o, e, dtype=dtype)
+    x, y, z = positions(seed, np_parcels, o, e, dtype=dtype)
+    d = np.full(np_parcels, 50e-6, dtype=dtype)
+    zero = np.zeros(np_parcels, dtype=dtype)
+    rho_c = 1.0
+    mu_c = 0.01 * rho_c  # nu = 0.01 m^2/s
+    return Case(n, o, e, U, d, x, y, z, zero, zero.copy(), zero.copy(), mu_c=mu_c, rho_d=2500.0,
+                rho_c=rho_c, dt_fluid=0.005, nsub=10, boundary=0, name=name)
