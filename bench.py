@@ -45,6 +45,8 @@ def parse():
                     help="time only the headline configuration (for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-parcels", type=int, default=1 << 22)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the streaming copy kernel over BYTES (traffic-counter calibration)")
     return ap.parse_args()
 
 
@@ -131,6 +133,11 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     ctx = stk.Context(0)
+    if args.calibrate_copy:
+        gbs = ctx.copy_bandwidth(args.calibrate_copy, reps=args.steps)
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "reps": args.steps, "GBs": gbs}))
+        return
+    copy_gbs = ctx.copy_bandwidth(1 << 30, reps=10)
     t0 = time.perf_counter()
     case = inputs.taylor_green_box(np_parcels=args.parcels, ncell=args.cells, boundary=stk.PERIODIC)
     print(f"# inputs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -183,6 +190,7 @@ def main():
             "kernel": "k_pass_sorted" if args.method == "sorted" else "k_pass_plain",
             "kernel_ms_per_pass": head["kernel_ms_per_pass"],
             "bytes_per_pass": bpp,
+            "copy_kernel_GBs": copy_gbs,
         },
         "stray_fraction": head["stray_fraction"],
     }

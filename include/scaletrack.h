@@ -155,6 +155,39 @@ int st_graph_destroy(st_graph *graph);
  * events recorded on the context stream around them; -1 if not recorded. */
 int st_last_kernel_ms(st_ctx *ctx, double *ms, int32_t *launches);
 
+/* ---- adjoint (FP64) -------------------------------------------------------------- */
+/* Extension: reverse-mode gradients of a loss of the deposited source (and optionally of
+ * the final parcel state) with respect to the per-parcel diameter, the per-parcel density
+ * and the drag-coefficient multiplier, through nsub substeps run as passes of
+ * checkpoint_every substeps with (x, v) saved before each pass and recomputed in the
+ * reverse sweep.  The derivative is the almost-everywhere one: cells, bounces and wraps
+ * are held at the forward run's.  Sequence: create, forward, seed (source misfit or
+ * cotangents), backward, gradients.  Host arrays are in upload order. */
+typedef struct st_adjoint st_adjoint;
+int st_adjoint_create(st_ctx *ctx, st_cloud *cloud, st_field *field, int32_t nsub,
+                      int32_t checkpoint_every, const st_params *params, st_adjoint **out);
+int st_adjoint_destroy(st_adjoint *adj);
+/* Zeroes UTrans and runs the forward step (substep = dt_fluid / nsub). */
+int st_adjoint_forward(st_adjoint *adj, double dt_fluid);
+/* loss = sum over cells and components of (UTrans - target)^2; seeds dL/dUTrans. */
+int st_adjoint_source_misfit(st_adjoint *adj, const double *target, double *loss);
+/* Seeds dL/dUTrans directly (AoS xyz, ncell*3). */
+int st_adjoint_set_source_cotangent(st_adjoint *adj, const double *dl_dsource);
+/* Seeds dL/d(final x, y, z) and dL/d(final u, v, w), SoA 3*n each; either may be NULL. */
+int st_adjoint_set_state_cotangent(st_adjoint *adj, const double *xbar, const double *vbar);
+int st_adjoint_backward(st_adjoint *adj);
+/* Any output may be NULL: grad_d, grad_rho (n each; rho needs per-parcel density),
+ * grad_cd (scalar), grad_x0 and grad_v0 (SoA 3*n, gradients of the initial state). */
+int st_adjoint_gradients(st_adjoint *adj, double *grad_d, double *grad_rho, double *grad_cd,
+                         double *grad_x0, double *grad_v0);
+
+/* ---- diagnostics -------------------------------------------------------------- */
+/* No counterpart in the reference.  Streams `bytes` from one device buffer to another
+ * `reps` times (16-byte loads and stores) and returns read + write GB/s: the measured
+ * copy bandwidth the parcel step's HBM figures are compared with, and the known-size
+ * kernel that hardware traffic counters are calibrated on. */
+int st_diag_copy_bandwidth(st_ctx *ctx, int64_t bytes, int32_t reps, double *gbs);
+
 #ifdef __cplusplus
 }
 #endif

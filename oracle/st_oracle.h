@@ -65,6 +65,20 @@ typedef struct st_oracle_args {
     int32_t substeps_per_pass;  /* substeps fused per pass (nsub = reference schedule) */
     int32_t nthreads;           /* 1 = the reference's serial loop; >1 OpenMP */
     double bbox[6];             /* out: min xyz, max xyz of the new positions */
+    /* Frozen events (gradient checks): np * nsub words, parcel-major.  events_mode
+     * 1 records each substep's cell, bounce and wrap decisions; 2 replays them instead
+     * of locating, so a perturbed run follows the recorded event sequence. */
+    int64_t *events;
+    int32_t events_mode;
+    /* Forward-mode dual numbers (FP64 only; active when tan_source != NULL): tangent
+     * seeds of the per-parcel diameter and density and of cd; tan_state (6*np, SoA
+     * x,y,z,u,v,w, may be NULL) carries state tangents in and out; the source tangent
+     * is added into tan_source (ncell*3). */
+    const double *tan_d;
+    const double *tan_rho;
+    double tan_cd;
+    double *tan_source;
+    double *tan_state;
 } st_oracle_args;
 
 int st_oracle_evolve(st_oracle_args *a);

@@ -38,6 +38,13 @@ class Args(C.Structure):
         ("substeps_per_pass", C.c_int32),
         ("nthreads", C.c_int32),
         ("bbox", C.c_double * 6),
+        ("events", C.c_void_p),
+        ("events_mode", C.c_int32),
+        ("tan_d", C.c_void_p),
+        ("tan_rho", C.c_void_p),
+        ("tan_cd", C.c_double),
+        ("tan_source", C.c_void_p),
+        ("tan_state", C.c_void_p),
     ]
 
 
@@ -70,9 +77,15 @@ def _p(a):
 
 def evolve(state, U, n, origin, ending, dt_fluid, nsub=10, precision=FP64, boundary=WALL,
            mu_c=1e-3, rho_d=1000.0, rho_d_by_rho_c=1.0, rho_c=1000.0, cd=1.0, rho=None,
-           source=None, substeps_per_pass=0, nthreads=1):
+           source=None, substeps_per_pass=0, nthreads=1, events=None, events_mode=0,
+           tan_d=None, tan_rho=None, tan_cd=0.0, tan_source=None, tan_state=None):
     """Advance `state` = [d, x, y, z, u, v, w] (numpy arrays, modified in place) and add
-    the momentum source into `source` (float64, (ncell, 3)).  Returns (source, bbox)."""
+    the momentum source into `source` (float64, (ncell, 3)).  Returns (source, bbox).
+
+    events: int64 array (np, nsub) recorded (events_mode=1) or replayed (2).
+    tan_source: float64 (ncell, 3); when given, the run is forward-mode dual numbers
+    seeded by tan_d, tan_rho (per parcel) and tan_cd, with state tangents in/out in
+    tan_state (6, np) if given."""
     dt = np.float64 if precision == FP64 else np.float32
     for a in state:
         assert a.dtype == dt and a.flags.c_contiguous
@@ -93,6 +106,22 @@ def evolve(state, U, n, origin, ending, dt_fluid, nsub=10, precision=FP64, bound
     a.mu_c, a.rho_d, a.rho_d_by_rho_c, a.rho_c, a.cd = mu_c, rho_d, rho_d_by_rho_c, rho_c, cd
     a.U, a.source = _p(U), _p(source)
     a.dt_fluid, a.nsub, a.substeps_per_pass, a.nthreads = dt_fluid, nsub, substeps_per_pass, nthreads
+    keep = []
+    if events is not None:
+        assert events.dtype == np.int64 and events.size == a.np * nsub and events.flags.c_contiguous
+        a.events, a.events_mode = _p(events), events_mode
+    if tan_source is not None:
+        assert tan_source.dtype == np.float64 and tan_source.size == 3 * ncell
+        for name, arr in (("tan_d", tan_d), ("tan_rho", tan_rho)):
+            if arr is not None:
+                arr = np.ascontiguousarray(arr, dtype=np.float64)
+                keep.append(arr)
+                setattr(a, name, _p(arr))
+        if tan_state is not None:
+            assert tan_state.dtype == np.float64 and tan_state.shape == (6, a.np) and tan_state.flags.c_contiguous
+            a.tan_state = _p(tan_state)
+        a.tan_cd = tan_cd
+        a.tan_source = _p(tan_source)
     r = lib.st_oracle_evolve(C.byref(a))
     if r != 0:
         raise ValueError("st_oracle_evolve rejected its arguments")

@@ -1,0 +1,295 @@
+// st_adjoint.hip -- reverse (adjoint) kernels of the parcel step, FP64.
+//
+// The loss L depends on the parcels through the deposited source S (and, for an
+// endpoint loss, the final positions and velocities).  Cells, bounces and wraps are
+// piecewise constant in the parameters, so the derivative is the almost-everywhere one
+// with the event sequence the forward run took.  Per substep and axis a, with
+// v' = (v + f u) / (1 + f), deposit D_a = m (v_a - v'_a) into the starting cell c,
+// g = dL/dS and adjoints vb (of v'), xb (of x, constant through the substeps):
+//   not bounced:  w = vb_a + xb_a dt - m g_a;   vb_a <- w / (1 + f) + m g_a
+//                 fbar += w (u_a - v'_a) / (1 + f);   mbar += g_a (v_a - v'_a)
+//   bounced:      v'_a = -v_a and D_a = 0, so vb_a <- -vb_a.
+// The deposit the reference makes per residence (asyncParallelTracking.jl:1342-1361,
+// with the bounce correction of :730-736) is the sum of these per-substep deposits,
+// so this is the derivative of exactly the forward's loss.
+//
+// Checkpoints: the forward saves (x, v) at the start of every segment of K substeps.
+// One launch per segment, last segment first.  Inside a segment each thread
+// recomputes its parcel forward from the checkpoint and keeps B substeps at a time in
+// registers (nested recompute: segment blocks of B are replayed last block first).
+#include "st_device.h"
+
+namespace st {
+
+struct AdjArgs {
+    const double *d, *rho;  // per-parcel diameter, density (rho may be null)
+    const double *ck;       // checkpoint of this segment: x, y, z, u, v, w (SoA, n each)
+    const double *U;        // cell velocity, AoS
+    const double *G;        // dL/dS, AoS
+    double *xb;             // adjoint of x, y, z (SoA, 3n)
+    double *vb;             // adjoint of u, v, w (SoA, 3n)
+    double *fbar, *mbar;    // accumulated adjoints of f and m per parcel
+    Geo g;
+    Phys<double> ph;
+    int32_t K;              // substeps in this segment
+    uint32_t np;
+};
+
+template <bool PERIODIC, int B>
+__global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.np) return;
+    const uint32_t n = a.np;
+    const double dia = a.d[p];
+    const double rho = a.rho ? a.rho[p] : a.ph.rho;
+    double m, f, den;
+    parcel_consts<double>(a.ph, a.rho != nullptr, dia, rho, m, f, den);
+    const double dt = a.ph.dt;
+
+    double xb0 = a.xb[p], xb1 = a.xb[n + p], xb2 = a.xb[2 * n + p];
+    double vb0 = a.vb[p], vb1 = a.vb[n + p], vb2 = a.vb[2 * n + p];
+    double fb = 0.0, mb = 0.0;
+
+    const double cx = a.ck[p], cy = a.ck[n + p], cz = a.ck[2 * n + p];
+    const double cu = a.ck[3 * n + p], cv = a.ck[4 * n + p], cw = a.ck[5 * n + p];
+
+    const int32_t nblk = (a.K + B - 1) / B;
+    for (int32_t blk = nblk - 1; blk >= 0; --blk) {
+        // replay from the checkpoint to the start of this block (as the forward pass of
+        // K substeps did: cell and velocity refreshed on each change before substep K)
+        double px = cx, py = cy, pz = cz, vx = cu, vy = cv, vz = cw;
+        int32_t I[3];
+        Cell cell;
+        locate<double, PERIODIC>(a.g, px, py, pz, I, cell);
+        const int32_t part0 = cell.part;
+        const double *q = a.U + 3u * cell.c;
+        double ux = q[0], uy = q[1], uz = q[2];
+        int32_t t = 1;
+        const int32_t t0 = blk * B;
+        for (; t <= t0; ++t) {
+            double nx, ny, nz, nvx, nvy, nvz;
+            Cell nc;
+            advance<double, PERIODIC>(a.g, a.ph, f, den, ux, uy, uz, px, py, pz, vx, vy, vz, cell, part0, nx,
+                                      ny, nz, nvx, nvy, nvz, nc);
+            if (t != a.K && nc.c != cell.c) {
+                cell = nc;
+                q = a.U + 3u * cell.c;
+                ux = q[0];
+                uy = q[1];
+                uz = q[2];
+            }
+            px = nx; py = ny; pz = nz;
+            vx = nvx; vy = nvy; vz = nvz;
+        }
+        // record this block
+        const int32_t nb = min(B, a.K - t0);
+        double rv[B][3];
+        uint32_t rc[B];
+        unsigned rb = 0;
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (j < nb) {
+                rv[j][0] = vx;
+                rv[j][1] = vy;
+                rv[j][2] = vz;
+                rc[j] = cell.c;
+                double nx, ny, nz, nvx, nvy, nvz;
+                Cell nc;
+                const unsigned bnc = advance<double, PERIODIC>(a.g, a.ph, f, den, ux, uy, uz, px, py, pz, vx, vy,
+                                                               vz, cell, part0, nx, ny, nz, nvx, nvy, nvz, nc);
+                rb |= bnc << (3 * j);
+                if (t != a.K && nc.c != cell.c) {
+                    cell = nc;
+                    q = a.U + 3u * cell.c;
+                    ux = q[0];
+                    uy = q[1];
+                    uz = q[2];
+                }
+                px = nx; py = ny; pz = nz;
+                vx = nvx; vy = nvy; vz = nvz;
+                ++t;
+            }
+        }
+        // reverse this block
+        double nvx = vx, nvy = vy, nvz = vz;  // velocity after the substep being reversed
+#pragma unroll
+        for (int j = B - 1; j >= 0; --j) {
+            if (j < nb) {
+                const uint32_t c = rc[j];
+                const double *uu = a.U + 3u * c;
+                const double *gg = a.G + 3u * c;
+                const unsigned bj = (rb >> (3 * j)) & 7u;
+                const double pv[3] = {rv[j][0], rv[j][1], rv[j][2]};
+                const double nv[3] = {nvx, nvy, nvz};
+                double *vbp[3] = {&vb0, &vb1, &vb2};
+                const double xbv[3] = {xb0, xb1, xb2};
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    if (bj & (1u << ax)) {
+                        *vbp[ax] = -*vbp[ax];
+                    } else {
+                        const double ga = gg[ax];
+                        const double w = *vbp[ax] + xbv[ax] * dt - m * ga;
+                        *vbp[ax] = w / den + m * ga;
+                        fb += w * (uu[ax] - nv[ax]) / den;
+                        mb += ga * (pv[ax] - nv[ax]);
+                    }
+                }
+                nvx = pv[0];
+                nvy = pv[1];
+                nvz = pv[2];
+            }
+        }
+    }
+    a.vb[p] = vb0;
+    a.vb[n + p] = vb1;
+    a.vb[2 * n + p] = vb2;
+    a.fbar[p] += fb;
+    a.mbar[p] += mb;
+}
+
+struct GradArgs {
+    const double *d, *rho;
+    const double *fbar, *mbar;
+    double *gd, *grho;  // per parcel (grho null with a chunk density)
+    double *gcd;        // scalar, accumulated
+    Phys<double> ph;
+    uint32_t np;
+};
+
+// d f/d d = -2 f / d, d f/d rho = -f / rho, d f/d cd = f / cd (= the factor without cd),
+// d m/d d = 3 m / d, d m/d rho = m / rho (per-parcel density only).
+__global__ void __launch_bounds__(256) k_adjoint_grads(GradArgs a) {
+    __shared__ double red[256 / kWave];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    double gc = 0.0;
+    if (p < a.np) {
+        const double dia = a.d[p];
+        const double rho = a.rho ? a.rho[p] : a.ph.rho;
+        double m, f, den;
+        parcel_consts<double>(a.ph, a.rho != nullptr, dia, rho, m, f, den);
+        const double fref = ((18.0 * a.ph.mu) * a.ph.dt) / (rho * (dia * dia));
+        const double fb = a.fbar[p], mb = a.mbar[p];
+        a.gd[p] = fb * (-2.0 * f / dia) + mb * (3.0 * m / dia);
+        if (a.grho) a.grho[p] = fb * (-f / rho) + mb * (m / rho);
+        gc = fb * fref;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) gc += __shfl_xor(gc, off, kWave);
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = gc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s += red[w];
+        atomic_add_global(a.gcd, s);
+    }
+}
+
+// G = 2 (S - T) and L += sum (S - T)^2 (the source-misfit loss).
+__global__ void __launch_bounds__(256) k_misfit(const double *S, const double *T, double *G, double *loss,
+                                                size_t n) {
+    __shared__ double red[256 / kWave];
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double r = S[i] - T[i];
+        G[i] = 2.0 * r;
+        acc += r * r;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s += red[w];
+        atomic_add_global(loss, s);
+    }
+}
+
+// dst[k*n + p] = src[k][p] for the six state arrays (a checkpoint).
+__global__ void k_save6(const double *x, const double *y, const double *z, const double *u, const double *v,
+                        const double *w, double *dst, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    dst[p] = x[p];
+    dst[n + p] = y[p];
+    dst[2 * n + p] = z[p];
+    dst[3 * n + p] = u[p];
+    dst[4 * n + p] = v[p];
+    dst[5 * n + p] = w[p];
+}
+
+// Device order <- upload order, k arrays of n: dst[j*n + p] = src[j*n + pid[p]].
+__global__ void k_gather_pid(const double *src, const int32_t *pid, double *dst, uint32_t n, int k) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t q = (uint32_t)pid[p];
+    for (int j = 0; j < k; ++j) dst[(size_t)j * n + p] = src[(size_t)j * n + q];
+}
+
+// Upload order <- device order: dst[pid[p]] = src[p].
+__global__ void k_scatter_pid(const double *src, const int32_t *pid, double *dst, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    dst[pid[p]] = src[p];
+}
+
+hipError_t launch_adjoint_segment(const AdjArgsHost &h, hipStream_t s) {
+    AdjArgs a;
+    a.d = h.d;
+    a.rho = h.rho;
+    a.ck = h.ck;
+    a.U = h.U;
+    a.G = h.G;
+    a.xb = h.xb;
+    a.vb = h.vb;
+    a.fbar = h.fbar;
+    a.mbar = h.mbar;
+    a.g = h.g;
+    a.ph = h.ph;
+    a.K = h.K;
+    a.np = h.np;
+    if (!a.np) return hipSuccess;
+    const unsigned grid = (a.np + 255) / 256;
+    if (h.periodic)
+        k_adjoint_segment<true, 10><<<grid, 256, 0, s>>>(a);
+    else
+        k_adjoint_segment<false, 10><<<grid, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_grads(const double *d, const double *rho, const double *fbar, const double *mbar,
+                                double *gd, double *grho, double *gcd, const Phys<double> &ph, uint32_t np,
+                                hipStream_t s) {
+    if (!np) return hipSuccess;
+    GradArgs a{d, rho, fbar, mbar, gd, grho, gcd, ph, np};
+    k_adjoint_grads<<<(np + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_misfit(const double *S, const double *T, double *G, double *loss, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    size_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    k_misfit<<<(unsigned)g, 256, 0, s>>>(S, T, G, loss, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_save6(const double *const f[6], double *dst, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_save6<<<(n + 255) / 256, 256, 0, s>>>(f[0], f[1], f[2], f[3], f[4], f[5], dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_pid(const double *src, const int32_t *pid, double *dst, uint32_t n, int k,
+                             hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_gather_pid<<<(n + 255) / 256, 256, 0, s>>>(src, pid, dst, n, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_pid(const double *src, const int32_t *pid, double *dst, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_scatter_pid<<<(n + 255) / 256, 256, 0, s>>>(src, pid, dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace st

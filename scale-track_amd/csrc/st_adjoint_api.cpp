@@ -1,0 +1,249 @@
+// st_adjoint_api.cpp -- C-ABI of the checkpointed adjoint of the parcel step (FP64).
+//
+// Forward: the parcel step over nsub substeps, run as passes of K substeps with the
+// parcel state (x, v) saved before each pass.  Backward: one reverse launch per pass,
+// last first, from the saved state (st_adjoint.hip).  Gradients of a loss of the
+// deposited source (and optionally of the final state) with respect to the per-parcel
+// diameter and density and to the drag-coefficient multiplier.
+#include "st_handles.h"
+
+#include <cmath>
+#include <new>
+#include <vector>
+
+using namespace st::host;
+
+struct st_adjoint {
+    st_ctx *ctx = nullptr;
+    st_cloud *cloud = nullptr;
+    st_field *field = nullptr;
+    st_params p{};
+    int32_t nsub = 0, K = 0, nseg = 0;
+    double dt_sub = 0.0;
+    uint32_t n = 0;
+    bool forward_done = false;
+    bool backward_done = false;
+    double *ck = nullptr;    // nseg * 6 * n
+    double *xb = nullptr;    // 3n
+    double *vb = nullptr;    // 3n, propagated by the backward sweep
+    double *vseed = nullptr; // 3n, the final-velocity cotangent the sweep starts from
+    double *fbar = nullptr;  // n
+    double *mbar = nullptr;  // n
+    double *G = nullptr;     // 3 * ncell
+    double *tmp = nullptr;   // 6n staging
+    double *scal = nullptr;  // [0] loss, [1] d L / d cd
+};
+
+namespace {
+
+int destroy(st_adjoint *a) {
+    if (!a) return ST_OK;
+    if (a->ctx) {
+        (void)hipSetDevice(a->ctx->device);
+        (void)hipStreamSynchronize(a->ctx->stream);
+    }
+    for (double *q : {a->ck, a->xb, a->vb, a->vseed, a->fbar, a->mbar, a->G, a->tmp, a->scal})
+        if (q) (void)hipFree(q);
+    delete a;
+    return ST_OK;
+}
+
+int check(st_adjoint *a) {
+    if (!a) return fail(ST_E_INVALID, "null adjoint handle");
+    return check_ctx(a->ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int st_adjoint_create(st_ctx *ctx, st_cloud *c, st_field *f, int32_t nsub, int32_t checkpoint_every,
+                      const st_params *params, st_adjoint **out) {
+    if (!out) return fail(ST_E_INVALID, "null output handle");
+    *out = nullptr;
+    st_params p;
+    ST_TRY(prepare(ctx, c, f, 1.0, nsub, params, p));
+    if (c->prec != ST_FP64) return fail(ST_E_UNSUPPORTED, "the adjoint runs in FP64; create the cloud with ST_FP64");
+    if (checkpoint_every < 1 || checkpoint_every > nsub)
+        return fail(ST_E_INVALID, "checkpoint_every must be in [1, nsub] (got %d)", checkpoint_every);
+    st_adjoint *a = new (std::nothrow) st_adjoint;
+    if (!a) return fail(ST_E_NOMEM, "host allocation failed");
+    a->ctx = ctx;
+    a->cloud = c;
+    a->field = f;
+    a->p = p;
+    a->p.substeps_per_pass = checkpoint_every;
+    a->nsub = nsub;
+    a->K = checkpoint_every;
+    a->nseg = (nsub + checkpoint_every - 1) / checkpoint_every;
+    a->n = (uint32_t)c->n;
+    const size_t n = a->n > 0 ? a->n : 1;
+    const size_t nc3 = (size_t)f->ncell * 3;
+    hipError_t e = hipMalloc((void **)&a->ck, (size_t)a->nseg * 6 * n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->xb, 3 * n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->vb, 3 * n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->vseed, 3 * n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->fbar, n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->mbar, n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->G, nc3 * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->tmp, 6 * n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->scal, 2 * 8);
+    if (e == hipSuccess) e = st::launch_fill_f64(a->G, 0.0, nc3, ctx->stream);
+    if (e != hipSuccess) {
+        destroy(a);
+        return fail(e == hipErrorOutOfMemory ? ST_E_NOMEM : ST_E_DEVICE, "adjoint allocation: %s",
+                    hipGetErrorString(e));
+    }
+    *out = a;
+    return ST_OK;
+}
+
+int st_adjoint_destroy(st_adjoint *a) { return destroy(a); }
+
+int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
+    ST_TRY(check(a));
+    st_cloud *c = a->cloud;
+    st_field *f = a->field;
+    st_params p;
+    ST_TRY(prepare(a->ctx, c, f, dt_fluid, a->nsub, &a->p, p));
+    hipStream_t s = a->ctx->stream;
+    if (p.method == ST_METHOD_SORTED && !sort_matches(c, f, p)) ST_TRY(sort_any(c, f, p));
+    ST_TRY(st_field_zero_source(f));
+    a->dt_sub = substep_dt(ST_FP64, dt_fluid, a->nsub);
+    const uint32_t n = a->n;
+    for (int32_t seg = 0; seg < a->nseg; ++seg) {
+        st::Soa<double> cur = soa<double>(c, c->cur);
+        const double *fields[6] = {cur.x, cur.y, cur.z, cur.u, cur.v, cur.w};
+        ST_HIP(st::launch_save6(fields, a->ck + (size_t)seg * 6 * n, n, s));
+        const int32_t ks = std::min(a->K, a->nsub - seg * a->K);
+        ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false));
+    }
+    ST_HIP(st::launch_fill_f64(a->xb, 0.0, 3 * (size_t)n, s));
+    ST_HIP(st::launch_fill_f64(a->vseed, 0.0, 3 * (size_t)n, s));
+    ST_HIP(st::launch_fill_f64(a->G, 0.0, (size_t)f->ncell * 3, s));
+    a->forward_done = true;
+    a->backward_done = false;
+    return ST_OK;
+}
+
+int st_adjoint_source_misfit(st_adjoint *a, const double *target, double *loss) {
+    ST_TRY(check(a));
+    if (!target || !loss) return fail(ST_E_INVALID, "null target or loss");
+    if (!a->forward_done) return fail(ST_E_STATE, "run st_adjoint_forward first");
+    st_field *f = a->field;
+    hipStream_t s = a->ctx->stream;
+    const size_t nc3 = (size_t)f->ncell * 3;
+    ST_TRY(merge_source(f));
+    double *T = nullptr;
+    ST_HIP(hipMalloc((void **)&T, nc3 * 8));
+    hipError_t e = hipMemcpyAsync(T, target, nc3 * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = st::launch_fill_f64(a->scal, 0.0, 1, s);
+    if (e == hipSuccess) e = st::launch_misfit(f->src, T, a->G, a->scal, nc3, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(loss, a->scal, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(T);
+    if (e != hipSuccess) return fail(ST_E_DEVICE, "source misfit: %s", hipGetErrorString(e));
+    return ST_OK;
+}
+
+int st_adjoint_set_source_cotangent(st_adjoint *a, const double *g) {
+    ST_TRY(check(a));
+    if (!g) return fail(ST_E_INVALID, "null cotangent");
+    if (!a->forward_done) return fail(ST_E_STATE, "run st_adjoint_forward first");
+    const size_t nc3 = (size_t)a->field->ncell * 3;
+    ST_HIP(hipMemcpyAsync(a->G, g, nc3 * 8, hipMemcpyHostToDevice, a->ctx->stream));
+    ST_HIP(hipStreamSynchronize(a->ctx->stream));
+    return ST_OK;
+}
+
+int st_adjoint_set_state_cotangent(st_adjoint *a, const double *xbar, const double *vbar) {
+    ST_TRY(check(a));
+    if (!a->forward_done) return fail(ST_E_STATE, "run st_adjoint_forward first");
+    hipStream_t s = a->ctx->stream;
+    const uint32_t n = a->n;
+    const int32_t *pid = a->cloud->pid[a->cloud->cur];
+    if (xbar) {
+        ST_HIP(hipMemcpyAsync(a->tmp, xbar, 3 * (size_t)n * 8, hipMemcpyHostToDevice, s));
+        ST_HIP(st::launch_gather_pid(a->tmp, pid, a->xb, n, 3, s));
+    }
+    if (vbar) {
+        ST_HIP(hipMemcpyAsync(a->tmp + 3 * (size_t)n, vbar, 3 * (size_t)n * 8, hipMemcpyHostToDevice, s));
+        ST_HIP(st::launch_gather_pid(a->tmp + 3 * (size_t)n, pid, a->vseed, n, 3, s));
+    }
+    ST_HIP(hipStreamSynchronize(s));
+    return ST_OK;
+}
+
+int st_adjoint_backward(st_adjoint *a) {
+    ST_TRY(check(a));
+    if (!a->forward_done) return fail(ST_E_STATE, "run st_adjoint_forward first");
+    st_cloud *c = a->cloud;
+    st_field *f = a->field;
+    hipStream_t s = a->ctx->stream;
+    const uint32_t n = a->n;
+    ST_HIP(st::launch_fill_f64(a->fbar, 0.0, n, s));
+    ST_HIP(st::launch_fill_f64(a->mbar, 0.0, n, s));
+    if (n) ST_HIP(hipMemcpyAsync(a->vb, a->vseed, 3 * (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    st::Soa<double> cur = soa<double>(c, c->cur);
+    st::AdjArgsHost h{};
+    h.d = cur.d;
+    h.rho = cur.rho;
+    h.U = (const double *)f->U;
+    h.G = a->G;
+    h.xb = a->xb;
+    h.vb = a->vb;
+    h.fbar = a->fbar;
+    h.mbar = a->mbar;
+    h.g = f->geo;
+    h.ph = make_phys<double>(c, f, a->dt_sub, a->p.cd);
+    h.np = n;
+    h.periodic = a->p.boundary == ST_PERIODIC;
+    ST_HIP(hipEventRecord(a->ctx->ev[0], s));
+    for (int32_t seg = a->nseg - 1; seg >= 0; --seg) {
+        h.ck = a->ck + (size_t)seg * 6 * n;
+        h.K = std::min(a->K, a->nsub - seg * a->K);
+        ST_HIP(st::launch_adjoint_segment(h, s));
+    }
+    ST_HIP(hipEventRecord(a->ctx->ev[1], s));
+    a->ctx->timed_launches = a->nseg;
+    ST_HIP(st::launch_fill_f64(a->scal + 1, 0.0, 1, s));
+    // gd -> tmp[0, n), grho -> tmp[n, 2n), device order
+    ST_HIP(st::launch_adjoint_grads(cur.d, cur.rho, a->fbar, a->mbar, a->tmp, cur.rho ? a->tmp + n : nullptr,
+                                    a->scal + 1, h.ph, n, s));
+    a->backward_done = true;
+    return ST_OK;
+}
+
+int st_adjoint_gradients(st_adjoint *a, double *grad_d, double *grad_rho, double *grad_cd, double *grad_x0,
+                         double *grad_v0) {
+    ST_TRY(check(a));
+    if (!a->backward_done) return fail(ST_E_STATE, "run st_adjoint_forward and st_adjoint_backward first");
+    st_cloud *c = a->cloud;
+    if (grad_rho && !c->has_rho) return fail(ST_E_INVALID, "grad_rho needs a cloud with per-parcel density");
+    hipStream_t s = a->ctx->stream;
+    const uint32_t n = a->n;
+    const int32_t *pid = c->pid[c->cur];
+    double *stage = a->tmp + 2 * (size_t)n;  // [2n, 3n) of the staging buffer
+    struct Item {
+        const double *src;
+        double *dst;
+    } items[8] = {{a->tmp, grad_d}, {a->tmp + n, grad_rho}};
+    int ni = 2;
+    if (grad_x0)
+        for (int k = 0; k < 3; ++k) items[ni++] = {a->xb + (size_t)k * n, grad_x0 + (size_t)k * n};
+    if (grad_v0)
+        for (int k = 0; k < 3; ++k) items[ni++] = {a->vb + (size_t)k * n, grad_v0 + (size_t)k * n};
+    for (int i = 0; i < ni; ++i) {
+        if (!items[i].dst || !n) continue;
+        ST_HIP(st::launch_scatter_pid(items[i].src, pid, stage, n, s));
+        ST_HIP(hipMemcpyAsync(items[i].dst, stage, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+        ST_HIP(hipStreamSynchronize(s));
+    }
+    if (grad_cd) {
+        ST_HIP(hipMemcpyAsync(grad_cd, a->scal + 1, 8, hipMemcpyDeviceToHost, s));
+        ST_HIP(hipStreamSynchronize(s));
+    }
+    return ST_OK;
+}
+
+}  // extern "C"

@@ -4,8 +4,7 @@
 // evolve_on_device! (run/icoJuliaParcelFoam/cavity3D/asyncParallelTracking.jl:1098-1125),
 // the UTrans fill! (:1064-1067), the U/UTrans copies (:1076-1083, :1148-1152) and the
 // bounding-box readback (:1106, :1113).  No status escapes as an exception.
-#include "../../include/scaletrack.h"
-#include "st_internal.h"
+#include "st_handles.h"
 
 #include <cmath>
 #include <cstdarg>
@@ -15,7 +14,8 @@
 #include <vector>
 #include <new>
 
-namespace {
+namespace st {
+namespace host {
 
 thread_local std::string g_err;
 
@@ -29,89 +29,7 @@ int fail(int code, const char *fmt, ...) {
     return code;
 }
 
-#define ST_HIP(call)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(e_ == hipErrorOutOfMemory ? ST_E_NOMEM : ST_E_DEVICE, "%s: %s (%s:%d)", \
-                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                  \
-    } while (0)
-
-#define ST_TRY(call)            \
-    do {                        \
-        int r_ = (call);        \
-        if (r_ != ST_OK) return r_; \
-    } while (0)
-
-constexpr int kNF = 8;  // d, x, y, z, u, v, w, rho
-
-}  // namespace
-
-struct st_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    int timed_launches = 0;  // launches between ev[0] and ev[1] of the last st_evolve
-};
-
-struct st_cloud {
-    st_ctx *ctx = nullptr;
-    int prec = ST_FP64;
-    int64_t n = 0;
-    size_t es = 8;
-    double mu_c = 0, rho_d = 0, rdr = 0, rho_c = 0;
-    int has_rho = 0;
-    void *buf[2][kNF] = {};
-    int32_t *pid[2] = {nullptr, nullptr};
-    int cur = 0;
-    bool uploaded = false;
-    bool fresh = false;     // uploaded since the last range check
-    bool permuted = false;  // device order differs from upload order
-    // sort state
-    uint32_t *keys = nullptr;
-    uint32_t *counts = nullptr;
-    uint32_t *cursor = nullptr;
-    int32_t *brick_start = nullptr;
-    int32_t bins_cap = 0;
-    bool sorted = false;
-    int32_t sort_n[3] = {0, 0, 0}, sort_bd[3] = {0, 0, 0}, sort_periodic = -1;
-    int32_t nb[3] = {0, 0, 0};
-    int32_t nbricks = 0;
-    double sort_o[3] = {0, 0, 0}, sort_e[3] = {0, 0, 0};
-    // statistics of the last launch sequence
-    unsigned long long *bbox = nullptr;  // device, 6 keys
-    unsigned int *strays = nullptr;      // device
-    unsigned int *bad = nullptr;         // device
-    unsigned int *strays_host = nullptr; // pinned
-    hipEvent_t strays_ev = nullptr;
-    bool strays_pending = false;
-    int64_t strays_den = 0;  // parcels x passes behind strays_host
-    double last_stray_fraction = -1.0;
-};
-
-struct st_field {
-    st_ctx *ctx = nullptr;
-    st_mesh mesh{};
-    st::Geo geo{};
-    int prec = ST_FP64;
-    size_t es = 8;
-    int64_t ncell = 0;
-    void *U = nullptr;
-    double *src = nullptr;
-    double *stray = nullptr;
-    bool U_set = false;
-    bool stray_dirty = false;
-};
-
-struct st_graph {
-    st_ctx *ctx = nullptr;
-    st_field *field = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    bool owner_flush = false;
-};
-
-namespace {
+const char *last_error_text() { return g_err.c_str(); }
 
 int check_ctx(st_ctx *ctx) {
     if (!ctx) return fail(ST_E_INVALID, "null context");
@@ -145,8 +63,8 @@ bool positive_all(const void *p, int64_t n, size_t es) {
     return true;
 }
 
-st::Geo make_geo(const st_mesh &m) {
-    st::Geo g{};
+Geo make_geo(const st_mesh &m) {
+    Geo g{};
     for (int c = 0; c < 3; ++c) {
         g.n[c] = m.n[c];
         g.o[c] = m.origin[c];
@@ -156,21 +74,6 @@ st::Geo make_geo(const st_mesh &m) {
     }
     g.nxny = (uint32_t)m.n[0] * (uint32_t)m.n[1];
     return g;
-}
-
-template <class T>
-st::Soa<T> soa(st_cloud *c, int which) {
-    st::Soa<T> s;
-    void **b = c->buf[which];
-    s.d = (T *)b[0];
-    s.x = (T *)b[1];
-    s.y = (T *)b[2];
-    s.z = (T *)b[3];
-    s.u = (T *)b[4];
-    s.v = (T *)b[5];
-    s.w = (T *)b[6];
-    s.rho = c->has_rho ? (const T *)b[7] : nullptr;
-    return s;
 }
 
 int alloc_set(st_cloud *c, int which) {
@@ -227,12 +130,12 @@ int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
     if (!c->keys) ST_HIP(hipMalloc((void **)&c->keys, (size_t)(c->n > 0 ? c->n : 1) * 4));
     ST_TRY(alloc_set(c, 1 - c->cur));
     const uint32_t np = (uint32_t)c->n;
-    ST_HIP(st::launch_fill_u32(c->counts, 0u, (size_t)nbr, s));
-    ST_HIP(st::launch_fill_u32(c->bad, 0u, 1, s));
-    ST_HIP(st::launch_brick_keys<T>(soa<T>(c, c->cur), np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC,
+    ST_HIP(launch_fill_u32(c->counts, 0u, (size_t)nbr, s));
+    ST_HIP(launch_fill_u32(c->bad, 0u, 1, s));
+    ST_HIP(launch_brick_keys<T>(soa<T>(c, c->cur), np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC,
                                     c->keys, c->counts, c->bad, s));
-    ST_HIP(st::launch_scan(c->counts, (int32_t)nbr, c->brick_start, c->cursor, s));
-    ST_HIP(st::launch_scatter<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur),
+    ST_HIP(launch_scan(c->counts, (int32_t)nbr, c->brick_start, c->cursor, s));
+    ST_HIP(launch_scatter<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur),
                                  c->pid[1 - c->cur], c->keys, c->cursor, np, s));
     c->cur = 1 - c->cur;
     c->permuted = true;
@@ -262,11 +165,11 @@ bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p) {
 
 int range_check(st_cloud *c, st_field *f) {
     hipStream_t s = c->ctx->stream;
-    ST_HIP(st::launch_fill_u32(c->bad, 0u, 1, s));
+    ST_HIP(launch_fill_u32(c->bad, 0u, 1, s));
     if (c->prec == ST_FP64)
-        ST_HIP(st::launch_range_check<double>(soa<double>(c, c->cur), (uint32_t)c->n, f->geo, c->bad, s));
+        ST_HIP(launch_range_check<double>(soa<double>(c, c->cur), (uint32_t)c->n, f->geo, c->bad, s));
     else
-        ST_HIP(st::launch_range_check<float>(soa<float>(c, c->cur), (uint32_t)c->n, f->geo, c->bad, s));
+        ST_HIP(launch_range_check<float>(soa<float>(c, c->cur), (uint32_t)c->n, f->geo, c->bad, s));
     unsigned int bad = 0;
     ST_HIP(hipMemcpyAsync(&bad, c->bad, 4, hipMemcpyDeviceToHost, s));
     ST_HIP(hipStreamSynchronize(s));
@@ -276,23 +179,17 @@ int range_check(st_cloud *c, st_field *f) {
 }
 
 template <class T>
-int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int nsub, const st_params &p,
+int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p,
                bool capturing) {
     hipStream_t s = ctx->stream;
-    st::PassArgs<T> a{};
+    PassArgs<T> a{};
     a.s = soa<T>(c, c->cur);
     a.U = (const T *)f->U;
     a.src = f->src;
     a.stray = f->stray;
     a.brick_start = c->brick_start;
     a.g = f->geo;
-    a.ph.dt = (T)dt_fluid / (T)nsub;  // Time stores dt as scalar (:272-275), then dt / nSteps (:1261)
-    a.ph.mu = (T)c->mu_c;
-    a.ph.rho = (T)c->rho_d;
-    a.ph.cd = (T)p.cd;
-    for (int k = 0; k < 3; ++k) a.ph.L[k] = (T)(f->mesh.ending[k] - f->mesh.origin[k]);
-    a.ph.rdr = c->rdr;
-    a.ph.rho_c = c->rho_c;
+    a.ph = make_phys<T>(c, f, dt_sub, p.cd);
     a.st.bbox = c->bbox;
     a.st.strays = c->strays;
     a.nbricks = c->nbricks;
@@ -306,15 +203,15 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int nsub,
     const int per = p.substeps_per_pass > 0 ? p.substeps_per_pass : nsub;
     const int periodic = p.boundary == ST_PERIODIC;
 
-    ST_HIP(st::launch_reset_stats(a.st, s));
+    ST_HIP(launch_reset_stats(a.st, s));
     if (!capturing) ST_HIP(hipEventRecord(ctx->ev[0], s));
     int launches = 0;
     for (int done = 0; done < nsub; done += per) {
         a.nsteps = per < nsub - done ? per : nsub - done;
         if (p.method == ST_METHOD_SORTED)
-            ST_HIP(st::launch_pass_sorted<T>(a, periodic, p.block, s));
+            ST_HIP(launch_pass_sorted<T>(a, periodic, p.block, s));
         else
-            ST_HIP(st::launch_pass_plain<T>(a, periodic, p.block, s));
+            ST_HIP(launch_pass_plain<T>(a, periodic, p.block, s));
         ++launches;
     }
     if (!capturing) {
@@ -361,13 +258,46 @@ int maybe_sort(st_cloud *c, st_field *f, const st_params &p) {
     return c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p);
 }
 
-}  // namespace
+
+int sort_any(st_cloud *c, st_field *f, const st_params &p) {
+    return c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p);
+}
+
+int merge_source(st_field *f) {
+    if (f->stray_dirty) {
+        ST_HIP(launch_merge(f->src, f->stray, (size_t)f->ncell * 3, f->ctx->stream));
+        f->stray_dirty = false;
+    }
+    return ST_OK;
+}
+
+template <class T>
+Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd) {
+    Phys<T> ph;
+    ph.dt = (T)dt_sub;
+    ph.mu = (T)c->mu_c;
+    ph.rho = (T)c->rho_d;
+    ph.cd = (T)cd;
+    for (int k = 0; k < 3; ++k) ph.L[k] = (T)(f->mesh.ending[k] - f->mesh.origin[k]);
+    ph.rdr = c->rdr;
+    ph.rho_c = c->rho_c;
+    return ph;
+}
+template Phys<float> make_phys<float>(const st_cloud *, const st_field *, double, double);
+template Phys<double> make_phys<double>(const st_cloud *, const st_field *, double, double);
+template int run_passes<float>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool);
+template int run_passes<double>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool);
+
+}  // namespace host
+}  // namespace st
+
+using namespace st::host;
 
 extern "C" {
 
 int st_version(void) { return ST_ABI_VERSION; }
 
-const char *st_last_error(void) { return g_err.c_str(); }
+const char *st_last_error(void) { return last_error_text(); }
 
 void st_params_default(st_params *p) {
     if (!p) return;
@@ -683,9 +613,9 @@ int st_evolve(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int32_t ns
     }
     if (p.method == ST_METHOD_SORTED) ST_TRY(maybe_sort(c, f, p));
     if (c->prec == ST_FP64)
-        ST_TRY(run_passes<double>(ctx, c, f, dt_fluid, nsub, p, false));
+        ST_TRY(run_passes<double>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, false));
     else
-        ST_TRY(run_passes<float>(ctx, c, f, dt_fluid, nsub, p, false));
+        ST_TRY(run_passes<float>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, false));
     if (bbox_out) {
         unsigned long long keys[6];
         ST_HIP(hipMemcpyAsync(keys, c->bbox, sizeof keys, hipMemcpyDeviceToHost, ctx->stream));
@@ -742,8 +672,8 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
         if (e != hipSuccess) r = fail(ST_E_DEVICE, "capture fill: %s", hipGetErrorString(e));
     }
     if (r == ST_OK)
-        r = c->prec == ST_FP64 ? run_passes<double>(ctx, c, f, dt_fluid, nsub, p, true)
-                               : run_passes<float>(ctx, c, f, dt_fluid, nsub, p, true);
+        r = c->prec == ST_FP64 ? run_passes<double>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, true)
+                               : run_passes<float>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, true);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(s, &graph);
     if (r != ST_OK || e != hipSuccess) {
@@ -792,6 +722,34 @@ int st_last_kernel_ms(st_ctx *ctx, double *ms, int32_t *launches) {
     float t = 0.f;
     ST_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
     *ms = (double)t / ctx->timed_launches;
+    return ST_OK;
+}
+
+int st_diag_copy_bandwidth(st_ctx *ctx, int64_t bytes, int32_t reps, double *gbs) {
+    ST_TRY(check_ctx(ctx));
+    if (!gbs || bytes < 16 || reps < 1) return fail(ST_E_INVALID, "need bytes >= 16, reps >= 1 and an output");
+    bytes &= ~(int64_t)15;
+    void *a = nullptr, *b = nullptr;
+    ST_HIP(hipMalloc(&a, (size_t)bytes));
+    hipError_t e = hipMalloc(&b, (size_t)bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return fail(ST_E_NOMEM, "copy buffers: %s", hipGetErrorString(e));
+    }
+    hipStream_t s = ctx->stream;
+    e = st::launch_fill_u32((uint32_t *)a, 0x3f800000u, (size_t)bytes / 4, s);
+    if (e == hipSuccess) e = st::launch_copy(a, b, (size_t)bytes, s);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = st::launch_copy(a, b, (size_t)bytes, s);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], s);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[1]);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(ST_E_DEVICE, "copy: %s", hipGetErrorString(e));
+    ctx->timed_launches = 0;
+    *gbs = 2.0 * (double)bytes * reps / ((double)ms * 1e-3) / 1e9;
     return ST_OK;
 }
 

@@ -1,0 +1,187 @@
+// st_device.h -- device functions shared by the forward and adjoint kernels: cell
+// location, wall fold, substep kinematics and per-parcel constants.  Both kernels call
+// the same code, so the adjoint's forward recompute takes exactly the forward's
+// decisions (cells, bounces, wraps).
+#pragma once
+
+#include "st_internal.h"
+
+namespace st {
+
+constexpr int kWave = 64;
+constexpr int kXcd = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+// Workgroup id -> brick id so that each XCD (and its L2) gets one contiguous range of
+// bricks instead of every 8th brick.
+__device__ __forceinline__ int32_t xcd_remap(int32_t bid, int32_t n) {
+    const int32_t q = n / kXcd, r = n % kXcd;
+    const int32_t x = bid % kXcd, j = bid / kXcd;
+    return x * q + (x < r ? x : r) + j;
+}
+
+__device__ __forceinline__ int32_t axis_floor(double x, double o, double rd) {
+    return (int32_t)floor((x - o) * rd);
+}
+
+// divrem by the partition size (= N for one partition) with truncation toward zero,
+// then abs() of both parts (asyncParallelTracking.jl:671-696).
+__device__ __forceinline__ void fold(int32_t I, int32_t N, int32_t &f, int32_t &p) {
+    if ((uint32_t)I < (uint32_t)N) {
+        f = I;
+        p = 0;
+    } else {
+        const int32_t q = I / N;
+        const int32_t r = I - q * N;
+        f = r < 0 ? -r : r;
+        p = q < 0 ? -q : q;
+    }
+}
+
+__device__ __forceinline__ int32_t clampi(int32_t i, int32_t n) {
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+struct Cell {
+    int32_t f[3];  // folded per-axis indices, always in [0, N)
+    uint32_t c;    // linear cell, x fastest
+    int32_t part;  // folded partition index (wall mode)
+};
+
+__device__ __forceinline__ void cell_wall(const Geo &g, const int32_t I[3], Cell &cl) {
+    int32_t p0, p1, p2;
+    fold(I[0], g.n[0], cl.f[0], p0);
+    fold(I[1], g.n[1], cl.f[1], p1);
+    fold(I[2], g.n[2], cl.f[2], p2);
+    cl.part = p0 + p1 + p2;
+    cl.c = (uint32_t)cl.f[2] * g.nxny + (uint32_t)cl.f[1] * (uint32_t)g.n[0] + (uint32_t)cl.f[0];
+}
+
+__device__ __forceinline__ void cell_clamped(const Geo &g, const int32_t I[3], Cell &cl) {
+    cl.f[0] = clampi(I[0], g.n[0]);
+    cl.f[1] = clampi(I[1], g.n[1]);
+    cl.f[2] = clampi(I[2], g.n[2]);
+    cl.part = 0;
+    cl.c = (uint32_t)cl.f[2] * g.nxny + (uint32_t)cl.f[1] * (uint32_t)g.n[0] + (uint32_t)cl.f[0];
+}
+
+template <class T, bool PERIODIC>
+__device__ __forceinline__ void locate(const Geo &g, T x, T y, T z, int32_t I[3], Cell &cl) {
+    I[0] = axis_floor((double)x, g.o[0], g.rd[0]);
+    I[1] = axis_floor((double)y, g.o[1], g.rd[1]);
+    I[2] = axis_floor((double)z, g.o[2], g.rd[2]);
+    if (PERIODIC)
+        cell_clamped(g, I, cl);
+    else
+        cell_wall(g, I, cl);
+}
+
+__device__ __forceinline__ void atomic_add_global(double *p, double v) {
+    unsafeAtomicAdd(p, v);  // global_atomic_add_f64 (coarse-grained hipMalloc memory)
+}
+__device__ __forceinline__ void atomic_add_lds(__attribute__((address_space(3))) double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The persistent state of one parcel as read at the start of a pass.
+template <class T>
+struct ParcelIn {
+    T d, x, y, z, u, v, w, rho;
+};
+
+template <class T>
+__device__ __forceinline__ void load_parcel(const Soa<T> &s, uint32_t p, T rho_chunk, ParcelIn<T> &in) {
+    in.d = s.d[p];
+    in.x = s.x[p];
+    in.y = s.y[p];
+    in.z = s.z[p];
+    in.u = s.u[p];
+    in.v = s.v[p];
+    in.w = s.w[p];
+    in.rho = s.rho ? s.rho[p] : rho_chunk;
+}
+
+// Kinematics of one substep (asyncParallelTracking.jl:1294-1335): implicit-Euler
+// drag update and advance, relocation, then the specular wall bounce -- only when the
+// folded cell or partition changed -- or the periodic wrap.  On return (n*) is the new
+// state, nc the cell after relocation, and the result has bit a set for each axis a
+// that bounced (the caller applies the source correction s0_a -= 2 v_a of :734).
+template <class T, bool PERIODIC>
+__device__ __forceinline__ unsigned advance(const Geo &g, const Phys<T> &ph, T f, T den, T ux, T uy,
+                                            T uz, T px, T py, T pz, T vx, T vy, T vz, const Cell &cell,
+                                            int32_t part0, T &nx, T &ny, T &nz, T &nvx, T &nvy, T &nvz,
+                                            Cell &nc) {
+    // :1295-1296
+    nvx = (vx + f * ux) / den;
+    nvy = (vy + f * uy) / den;
+    nvz = (vz + f * uz) / den;
+    nx = px + nvx * ph.dt;
+    ny = py + nvy * ph.dt;
+    nz = pz + nvz * ph.dt;
+    int32_t I[3];
+    unsigned bounced = 0;
+    if (!PERIODIC) {
+        locate<T, false>(g, nx, ny, nz, I, nc);  // :1303
+        if (nc.c != cell.c || nc.part != part0) {  // :1306
+            if (I[0] < 0 || I[0] >= g.n[0]) {  // :1309-1329
+                nx = px;
+                nvx = -vx;
+                bounced |= 1u;
+            }
+            if (I[1] < 0 || I[1] >= g.n[1]) {
+                ny = py;
+                nvy = -vy;
+                bounced |= 2u;
+            }
+            if (I[2] < 0 || I[2] >= g.n[2]) {
+                nz = pz;
+                nvz = -vz;
+                bounced |= 4u;
+            }
+            if (bounced) locate<T, false>(g, nx, ny, nz, I, nc);  // :1333-1335
+        }
+    } else {
+        // periodic extension: wrap each axis that left [0, N) by one period
+        I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
+        I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
+        I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
+        if (I[0] < 0) {
+            nx = nx + ph.L[0];
+            I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
+        } else if (I[0] >= g.n[0]) {
+            nx = nx - ph.L[0];
+            I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
+        }
+        if (I[1] < 0) {
+            ny = ny + ph.L[1];
+            I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
+        } else if (I[1] >= g.n[1]) {
+            ny = ny - ph.L[1];
+            I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
+        }
+        if (I[2] < 0) {
+            nz = nz + ph.L[2];
+            I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
+        } else if (I[2] >= g.n[2]) {
+            nz = nz - ph.L[2];
+            I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
+        }
+        cell_clamped(g, I, nc);
+    }
+    return bounced;
+}
+
+// Per-parcel constants (:1280 and :1294): mass over carrier density m (double in both
+// precision forms), drag factor f = cd * ((18 mu) dt) / (rho d^2) and 1 + f.  f does
+// not depend on position or velocity, so one value serves every substep.
+template <class T>
+__device__ __forceinline__ void parcel_consts(const Phys<T> &ph, bool per_parcel_rho, T dia, T rho,
+                                              double &m, T &f, T &den) {
+    const double rdr = per_parcel_rho ? (double)rho / ph.rho_c : ph.rdr;
+    const double dd = (double)dia;
+    m = ((rdr * kPi) * ((dd * dd) * dd)) / 6.0;
+    f = (((T)18 * ph.mu) * ph.dt) / (rho * (dia * dia));
+    f = ph.cd * f;
+    den = (T)1 + f;
+}
+
+}  // namespace st

@@ -1,0 +1,139 @@
+// st_handles.h -- host-side handle types and helpers shared by the C-ABI sources.
+#pragma once
+
+#include "../../include/scaletrack.h"
+#include "st_internal.h"
+
+#include <string>
+
+namespace st {
+namespace host {
+int fail(int code, const char *fmt, ...);
+}  // namespace host
+}  // namespace st
+
+#define ST_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return ::st::host::fail(e_ == hipErrorOutOfMemory ? ST_E_NOMEM : ST_E_DEVICE,         \
+                                    "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+#define ST_TRY(call)                \
+    do {                            \
+        int r_ = (call);            \
+        if (r_ != ST_OK) return r_; \
+    } while (0)
+
+namespace st {
+namespace host {
+constexpr int kNF = 8;  // d, x, y, z, u, v, w, rho
+}  // namespace host
+}  // namespace st
+
+struct st_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int timed_launches = 0;  // launches between ev[0] and ev[1] of the last st_evolve
+};
+
+struct st_cloud {
+    st_ctx *ctx = nullptr;
+    int prec = ST_FP64;
+    int64_t n = 0;
+    size_t es = 8;
+    double mu_c = 0, rho_d = 0, rdr = 0, rho_c = 0;
+    int has_rho = 0;
+    void *buf[2][st::host::kNF] = {};
+    int32_t *pid[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool uploaded = false;
+    bool fresh = false;     // uploaded since the last range check
+    bool permuted = false;  // device order differs from upload order
+    // sort state
+    uint32_t *keys = nullptr;
+    uint32_t *counts = nullptr;
+    uint32_t *cursor = nullptr;
+    int32_t *brick_start = nullptr;
+    int32_t bins_cap = 0;
+    bool sorted = false;
+    int32_t sort_n[3] = {0, 0, 0}, sort_bd[3] = {0, 0, 0}, sort_periodic = -1;
+    int32_t nb[3] = {0, 0, 0};
+    int32_t nbricks = 0;
+    double sort_o[3] = {0, 0, 0}, sort_e[3] = {0, 0, 0};
+    // statistics of the last launch sequence
+    unsigned long long *bbox = nullptr;  // device, 6 keys
+    unsigned int *strays = nullptr;      // device
+    unsigned int *bad = nullptr;         // device
+    unsigned int *strays_host = nullptr; // pinned
+    hipEvent_t strays_ev = nullptr;
+    bool strays_pending = false;
+    int64_t strays_den = 0;  // parcels x passes behind strays_host
+    double last_stray_fraction = -1.0;
+};
+
+struct st_field {
+    st_ctx *ctx = nullptr;
+    st_mesh mesh{};
+    st::Geo geo{};
+    int prec = ST_FP64;
+    size_t es = 8;
+    int64_t ncell = 0;
+    void *U = nullptr;
+    double *src = nullptr;
+    double *stray = nullptr;
+    bool U_set = false;
+    bool stray_dirty = false;
+};
+
+struct st_graph {
+    st_ctx *ctx = nullptr;
+    st_field *field = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool owner_flush = false;
+};
+
+
+namespace st {
+namespace host {
+
+const char *last_error_text();
+int check_ctx(st_ctx *ctx);
+int alloc_set(st_cloud *c, int which);
+int validate_params(const st_params &p);
+bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p);
+int sort_any(st_cloud *c, st_field *f, const st_params &p);
+int prepare(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int nsub, const st_params *pin, st_params &p);
+// Queues nsub substeps of length dt_sub (exact in T) on the context stream, split in
+// passes of p.substeps_per_pass.
+template <class T>
+int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p, bool capturing);
+// The substep length the reference uses: Time stores dt as `scalar` (:272-275) and the
+// kernel divides by nSteps (:1261).
+inline double substep_dt(int prec, double dt_fluid, int nsub) {
+    return prec == ST_FP64 ? dt_fluid / (double)nsub : (double)((float)dt_fluid / (float)nsub);
+}
+template <class T>
+inline Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd);
+int merge_source(st_field *f);
+
+template <class T>
+inline Soa<T> soa(st_cloud *c, int which) {
+    Soa<T> s;
+    void **b = c->buf[which];
+    s.d = (T *)b[0];
+    s.x = (T *)b[1];
+    s.y = (T *)b[2];
+    s.z = (T *)b[3];
+    s.u = (T *)b[4];
+    s.v = (T *)b[5];
+    s.w = (T *)b[6];
+    s.rho = c->has_rho ? (const T *)b[7] : nullptr;
+    return s;
+}
+
+}  // namespace host
+}  // namespace st

@@ -105,6 +105,27 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s)
 hipError_t launch_fill_f64(double *p, double value, size_t n, hipStream_t s);
 hipError_t launch_iota(int32_t *p, uint32_t n, hipStream_t s);
 hipError_t launch_merge(double *src, double *stray, size_t n, hipStream_t s);
+hipError_t launch_copy(const void *a, void *b, size_t bytes, hipStream_t s);
+
+// Adjoint (st_adjoint.hip), FP64.
+struct AdjArgsHost {
+    const double *d, *rho, *ck, *U, *G;
+    double *xb, *vb, *fbar, *mbar;
+    Geo g;
+    Phys<double> ph;
+    int32_t K;
+    uint32_t np;
+    int periodic;
+};
+hipError_t launch_adjoint_segment(const AdjArgsHost &h, hipStream_t s);
+hipError_t launch_adjoint_grads(const double *d, const double *rho, const double *fbar, const double *mbar,
+                                double *gd, double *grho, double *gcd, const Phys<double> &ph, uint32_t np,
+                                hipStream_t s);
+hipError_t launch_misfit(const double *S, const double *T, double *G, double *loss, size_t n, hipStream_t s);
+hipError_t launch_save6(const double *const f[6], double *dst, uint32_t n, hipStream_t s);
+hipError_t launch_gather_pid(const double *src, const int32_t *pid, double *dst, uint32_t n, int k,
+                             hipStream_t s);
+hipError_t launch_scatter_pid(const double *src, const int32_t *pid, double *dst, uint32_t n, hipStream_t s);
 template <class T>
 hipError_t launch_range_check(const Soa<T> &s, uint32_t np, Geo g, unsigned int *bad,
                               hipStream_t st);

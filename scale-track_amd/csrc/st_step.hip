@@ -14,83 +14,9 @@
 //    brick's cell velocities in LDS, sums its deposits in LDS (FP64 ds_add), and adds
 //    them to HBM once per cell.  Parcels that drifted out of their brick fall back to
 //    HBM gathers and FP64 atomics into a separate array.
-#include "st_internal.h"
+#include "st_device.h"
 
 namespace st {
-
-constexpr int kWave = 64;
-constexpr int kXcd = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
-
-// Workgroup id -> brick id so that each XCD (and its L2) gets one contiguous range of
-// bricks instead of every 8th brick.
-__device__ __forceinline__ int32_t xcd_remap(int32_t bid, int32_t n) {
-    const int32_t q = n / kXcd, r = n % kXcd;
-    const int32_t x = bid % kXcd, j = bid / kXcd;
-    return x * q + (x < r ? x : r) + j;
-}
-
-__device__ __forceinline__ int32_t axis_floor(double x, double o, double rd) {
-    return (int32_t)floor((x - o) * rd);
-}
-
-// divrem by the partition size (= N for one partition) with truncation toward zero,
-// then abs() of both parts (asyncParallelTracking.jl:671-696).
-__device__ __forceinline__ void fold(int32_t I, int32_t N, int32_t &f, int32_t &p) {
-    if ((uint32_t)I < (uint32_t)N) {
-        f = I;
-        p = 0;
-    } else {
-        const int32_t q = I / N;
-        const int32_t r = I - q * N;
-        f = r < 0 ? -r : r;
-        p = q < 0 ? -q : q;
-    }
-}
-
-__device__ __forceinline__ int32_t clampi(int32_t i, int32_t n) {
-    return i < 0 ? 0 : (i >= n ? n - 1 : i);
-}
-
-struct Cell {
-    int32_t f[3];  // folded per-axis indices, always in [0, N)
-    uint32_t c;    // linear cell, x fastest
-    int32_t part;  // folded partition index (wall mode)
-};
-
-__device__ __forceinline__ void cell_wall(const Geo &g, const int32_t I[3], Cell &cl) {
-    int32_t p0, p1, p2;
-    fold(I[0], g.n[0], cl.f[0], p0);
-    fold(I[1], g.n[1], cl.f[1], p1);
-    fold(I[2], g.n[2], cl.f[2], p2);
-    cl.part = p0 + p1 + p2;
-    cl.c = (uint32_t)cl.f[2] * g.nxny + (uint32_t)cl.f[1] * (uint32_t)g.n[0] + (uint32_t)cl.f[0];
-}
-
-__device__ __forceinline__ void cell_clamped(const Geo &g, const int32_t I[3], Cell &cl) {
-    cl.f[0] = clampi(I[0], g.n[0]);
-    cl.f[1] = clampi(I[1], g.n[1]);
-    cl.f[2] = clampi(I[2], g.n[2]);
-    cl.part = 0;
-    cl.c = (uint32_t)cl.f[2] * g.nxny + (uint32_t)cl.f[1] * (uint32_t)g.n[0] + (uint32_t)cl.f[0];
-}
-
-template <class T, bool PERIODIC>
-__device__ __forceinline__ void locate(const Geo &g, T x, T y, T z, int32_t I[3], Cell &cl) {
-    I[0] = axis_floor((double)x, g.o[0], g.rd[0]);
-    I[1] = axis_floor((double)y, g.o[1], g.rd[1]);
-    I[2] = axis_floor((double)z, g.o[2], g.rd[2]);
-    if (PERIODIC)
-        cell_clamped(g, I, cl);
-    else
-        cell_wall(g, I, cl);
-}
-
-__device__ __forceinline__ void atomic_add_global(double *p, double v) {
-    unsafeAtomicAdd(p, v);  // global_atomic_add_f64 (coarse-grained hipMalloc memory)
-}
-__device__ __forceinline__ void atomic_add_lds(double *p, double v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 
 // Everything in HBM: the plain method.
 template <class T>
@@ -112,13 +38,17 @@ struct GlobalEnv {
     }
 };
 
+// LDS-qualified pointers: reads through them are ds_read, never flat loads (a flat
+// load of the tile would share the vector-memory counter with the HBM loads in flight).
+#define ST_LDS __attribute__((address_space(3)))
+
 // A brick staged in LDS; cells outside it fall back to HBM.
 template <class T>
 struct TileEnv {
     const T *__restrict__ U;
     double *__restrict__ out;  // stray array (owner flush) or the source (atomic flush)
-    const T *Ut;               // LDS velocity tile, AoS
-    double *St;                // LDS source tile, AoS
+    const ST_LDS T *Ut;        // LDS velocity tile, AoS
+    ST_LDS double *St;         // LDS source tile, AoS
     int32_t o0, o1, o2;        // tile origin (cells)
     uint32_t e0, e1, e2;       // tile extent (cells)
     uint32_t strays;
@@ -151,7 +81,7 @@ struct TileEnv {
     __device__ __forceinline__ void deposit(const Cell &cl, double a, double b, double c) {
         uint32_t l;
         if (local(cl, l)) {
-            double *q = St + 3u * l;
+            ST_LDS double *q = St + 3u * l;
             atomic_add_lds(q, a);
             atomic_add_lds(q + 1, b);
             atomic_add_lds(q + 2, c);
@@ -183,23 +113,24 @@ struct Box {
     }
 };
 
-// One parcel, nsteps substeps (asyncParallelTracking.jl:1271-1365).
-template <class T, bool PERIODIC, class Env>
-__device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
-                                            uint32_t p, int32_t nsteps, Env &env, Box<T> &box) {
-    const T dia = s.d[p];
-    T rho = ph.rho;
-    double rdr = ph.rdr;
-    if (s.rho) {
-        rho = s.rho[p];
-        rdr = (double)rho / ph.rho_c;
-    }
-    // :1280 mass over carrier density, in double in both precision forms
-    const double dd = (double)dia;
-    const double m = ((rdr * kPi) * ((dd * dd) * dd)) / 6.0;
+struct NoPrefetch {
+    __device__ __forceinline__ void operator()() const {}
+};
 
-    T px = s.x[p], py = s.y[p], pz = s.z[p];
-    T vx = s.u[p], vy = s.v[p], vz = s.w[p];
+// One parcel, nsteps substeps (asyncParallelTracking.jl:1271-1365).  `prefetch` runs
+// once the parcel's starting cell velocity has been read, so loads it issues (the next
+// parcel's state) stay in flight across the substeps: vector-memory loads complete in
+// order, and a later HBM gather would otherwise wait for them.
+template <class T, bool PERIODIC, class Env, class Prefetch>
+__device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
+                                            const ParcelIn<T> &in, uint32_t p, int32_t nsteps,
+                                            Env &env, Box<T> &box, const Prefetch &prefetch) {
+    double m;
+    T f, den;
+    parcel_consts(ph, s.rho != nullptr, in.d, in.rho, m, f, den);
+
+    T px = in.x, py = in.y, pz = in.z;
+    T vx = in.u, vy = in.v, vz = in.w;
 
     int32_t I[3];
     Cell cell;
@@ -209,91 +140,26 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     T ux, uy, uz;
     env.load_u(cell, ux, uy, uz);  // :1287-1288
     double s0x = (double)vx, s0y = (double)vy, s0z = (double)vz;  // :1289
-
-    // :1294 -- the drag factor depends on neither position nor velocity, so it is the
-    // same value in every substep.
-    T f = (((T)18 * ph.mu) * ph.dt) / (rho * (dia * dia));
-    f = ph.cd * f;
-    const T den = (T)1 + f;
+    // Consume the velocity before the prefetch so its wait cannot cover the new loads.
+    __builtin_amdgcn_s_waitcnt(0);
+    prefetch();
 
     for (int32_t t = 1; t <= nsteps; ++t) {
-        // :1295-1296
-        T nvx = (vx + f * ux) / den;
-        T nvy = (vy + f * uy) / den;
-        T nvz = (vz + f * uz) / den;
-        T nx = px + nvx * ph.dt;
-        T ny = py + nvy * ph.dt;
-        T nz = pz + nvz * ph.dt;
+        T nx, ny, nz, nvx, nvy, nvz;
         Cell nc;
-        if (!PERIODIC) {
-            locate<T, false>(g, nx, ny, nz, I, nc);  // :1303
-            if (nc.c != cell.c || nc.part != part0) {  // :1306
-                bool hit = false;
-                if (I[0] < 0 || I[0] >= g.n[0]) {  // :1309-1329
-                    nx = px;
-                    nvx = -vx;
-                    s0x = s0x - 2.0 * (double)vx;
-                    hit = true;
-                }
-                if (I[1] < 0 || I[1] >= g.n[1]) {
-                    ny = py;
-                    nvy = -vy;
-                    s0y = s0y - 2.0 * (double)vy;
-                    hit = true;
-                }
-                if (I[2] < 0 || I[2] >= g.n[2]) {
-                    nz = pz;
-                    nvz = -vz;
-                    s0z = s0z - 2.0 * (double)vz;
-                    hit = true;
-                }
-                if (hit) locate<T, false>(g, nx, ny, nz, I, nc);  // :1333-1335
-                if (t != nsteps && nc.c != cell.c) {                // :1342-1348
-                    env.deposit(cell, m * (s0x - (double)nvx), m * (s0y - (double)nvy),
-                                m * (s0z - (double)nvz));
-                    s0x = (double)nvx;
-                    s0y = (double)nvy;
-                    s0z = (double)nvz;
-                    cell = nc;
-                    env.load_u(cell, ux, uy, uz);
-                }
-            }
-        } else {
-            // periodic extension: wrap each axis that left [0, N) by one period
-            I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
-            I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
-            I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
-            if (I[0] < 0) {
-                nx = nx + ph.L[0];
-                I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
-            } else if (I[0] >= g.n[0]) {
-                nx = nx - ph.L[0];
-                I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
-            }
-            if (I[1] < 0) {
-                ny = ny + ph.L[1];
-                I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
-            } else if (I[1] >= g.n[1]) {
-                ny = ny - ph.L[1];
-                I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
-            }
-            if (I[2] < 0) {
-                nz = nz + ph.L[2];
-                I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
-            } else if (I[2] >= g.n[2]) {
-                nz = nz - ph.L[2];
-                I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
-            }
-            cell_clamped(g, I, nc);
-            if (t != nsteps && nc.c != cell.c) {
-                env.deposit(cell, m * (s0x - (double)nvx), m * (s0y - (double)nvy),
-                            m * (s0z - (double)nvz));
-                s0x = (double)nvx;
-                s0y = (double)nvy;
-                s0z = (double)nvz;
-                cell = nc;
-                env.load_u(cell, ux, uy, uz);
-            }
+        const unsigned b = advance<T, PERIODIC>(g, ph, f, den, ux, uy, uz, px, py, pz, vx, vy, vz, cell,
+                                                part0, nx, ny, nz, nvx, nvy, nvz, nc);
+        if (b & 1u) s0x = s0x - 2.0 * (double)vx;  // update_at_boundary!, :730-736
+        if (b & 2u) s0y = s0y - 2.0 * (double)vy;
+        if (b & 4u) s0z = s0z - 2.0 * (double)vz;
+        if (t != nsteps && nc.c != cell.c) {  // :1342-1348
+            env.deposit(cell, m * (s0x - (double)nvx), m * (s0y - (double)nvy), m * (s0z - (double)nvz));
+            s0x = (double)nvx;
+            s0y = (double)nvy;
+            s0z = (double)nvz;
+            cell = nc;
+            env.load_u(cell, ux, uy, uz);
+            __builtin_amdgcn_s_waitcnt(0);  // wait here, not at every substep
         }
         vx = nvx;  // :1351-1352
         vy = nvy;
@@ -362,7 +228,11 @@ __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
     Box<T> box;
     box.init();
     GlobalEnv<T> env{a.U, a.src};
-    if (p < a.np) step_parcel<T, PERIODIC>(a.g, a.ph, a.s, p, a.nsteps, env, box);
+    if (p < a.np) {
+        ParcelIn<T> in;
+        load_parcel(a.s, p, a.ph.rho, in);
+        step_parcel<T, PERIODIC>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
+    }
     flush_stats(box, 0u, a.st);
 }
 
@@ -388,8 +258,8 @@ __global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
     env.e2 = (uint32_t)ez;
     env.strays = 0;
     const int32_t ncell = ex * ey * ez;
-    double *St = lds;
-    T *Ut = (T *)(lds + 3 * a.tile_cells);
+    ST_LDS double *St = (ST_LDS double *)lds;
+    ST_LDS T *Ut = (ST_LDS T *)(St + 3 * a.tile_cells);
     env.St = St;
     env.Ut = Ut;
 
@@ -414,8 +284,17 @@ __global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
     Box<T> box;
     box.init();
     const int32_t p0 = a.brick_start[brick], p1 = a.brick_start[brick + 1];
-    for (int32_t p = p0 + (int32_t)threadIdx.x; p < p1; p += blockDim.x)
-        step_parcel<T, PERIODIC>(a.g, a.ph, a.s, (uint32_t)p, a.nsteps, env, box);
+    // Software pipeline: the next parcel's state is in flight while this one steps.
+    int32_t p = p0 + (int32_t)threadIdx.x;
+    ParcelIn<T> cur, nxt;
+    if (p < p1) load_parcel(a.s, (uint32_t)p, a.ph.rho, cur);
+    for (; p < p1; p += blockDim.x) {
+        const int32_t q = p + (int32_t)blockDim.x;
+        step_parcel<T, PERIODIC>(a.g, a.ph, a.s, cur, (uint32_t)p, a.nsteps, env, box, [&] {
+            if (q < p1) load_parcel(a.s, (uint32_t)q, a.ph.rho, nxt);
+        });
+        cur = nxt;
+    }
     __syncthreads();
 
     // Add the brick's sums to HBM once per cell.  With the owner flush only this
@@ -714,5 +593,20 @@ template hipError_t launch_range_check<float>(const Soa<float> &, uint32_t, Geo,
                                               hipStream_t);
 template hipError_t launch_range_check<double>(const Soa<double> &, uint32_t, Geo, unsigned int *,
                                                hipStream_t);
+
+// Streaming copy (bandwidth calibration): 16-byte loads and stores, grid-stride.
+__global__ void __launch_bounds__(256) k_copy(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+hipError_t launch_copy(const void *a, void *b, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 16;
+    if (!n) return hipSuccess;
+    size_t g = (n + 255) / 256;
+    if (g > 256 * 64) g = 256 * 64;
+    k_copy<<<(unsigned)g, 256, 0, s>>>((const uint4 *)a, (uint4 *)b, n);
+    return hipGetLastError();
+}
 
 }  // namespace st

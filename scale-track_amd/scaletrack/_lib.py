@@ -93,6 +93,15 @@ SIGNATURES = {
     "st_graph_launch": (C.c_int, [_vp]),
     "st_graph_destroy": (C.c_int, [_vp]),
     "st_last_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "st_diag_copy_bandwidth": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "st_adjoint_create": (C.c_int, [_vp, _vp, _vp, C.c_int32, C.c_int32, C.POINTER(st_params), _pp]),
+    "st_adjoint_destroy": (C.c_int, [_vp]),
+    "st_adjoint_forward": (C.c_int, [_vp, C.c_double]),
+    "st_adjoint_source_misfit": (C.c_int, [_vp, _vp, C.POINTER(C.c_double)]),
+    "st_adjoint_set_source_cotangent": (C.c_int, [_vp, _vp]),
+    "st_adjoint_set_state_cotangent": (C.c_int, [_vp, _vp, _vp]),
+    "st_adjoint_backward": (C.c_int, [_vp]),
+    "st_adjoint_gradients": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_double), _vp, _vp]),
 }
 
 

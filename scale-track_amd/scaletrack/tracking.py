@@ -53,6 +53,12 @@ class Context:
         check(lib.st_last_kernel_ms(self.handle, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def copy_bandwidth(self, nbytes=1 << 30, reps=10):
+        """Measured device-to-device streaming copy, read + write GB/s."""
+        g = C.c_double()
+        check(lib.st_diag_copy_bandwidth(self.handle, int(nbytes), int(reps), C.byref(g)))
+        return g.value
+
     def close(self):
         if self.handle:
             lib.st_ctx_destroy(self.handle)

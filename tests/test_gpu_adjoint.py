@@ -1,0 +1,149 @@
+"""GPU adjoint (st_adjoint_*) against the oracle's forward-mode dual numbers and
+against central finite differences (eps = 1e-6, FP64) with the event sequence frozen.
+
+Bars: adjoint vs dual numbers 1e-9 relative (same derivative, different summation
+order); adjoint vs frozen-event finite differences 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+import scaletrack as stk
+from scaletrack.adjoint import Adjoint, source_misfit_rrule
+from oracle import st_oracle
+from scaletrack import inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def make_case(np_parcels=3000, ncell=12, boundary=1, nsub=40, seed=7):
+    n = (ncell, ncell, ncell)
+    o, e = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    U = inputs.fourier_field(n, o, e, modes=8, seed=seed)
+    x, y, z = inputs.positions(seed, np_parcels, o, e)
+    d = inputs.lognormal_diameters(seed, np_parcels, median=60e-6)
+    rho = 1000.0 * (1.0 + 0.5 * inputs.uniform(seed, 50, np_parcels))
+    zero = np.zeros(np_parcels)
+    return inputs.Case(n, o, e, U, d, x, y, z, zero, zero.copy(), zero.copy(), rho=rho, mu_c=1e-3,
+                       rho_d=1000.0, rho_c=1000.0, dt_fluid=nsub * 1e-4, nsub=nsub, boundary=boundary)
+
+
+def target(case):
+    return 1e-11 * inputs.normal(11, 60, case.ncell * 3).reshape(case.ncell, 3)
+
+
+def gpu_objects(ctx, case, cd=1.0):
+    mesh = stk.Mesh(case.n, case.origin, case.ending)
+    field = stk.TwoWayEulerian(ctx, mesh, stk.FP64)
+    field.upload_U(case.U)
+    chunk = stk.Chunk(ctx, case.np, case.mu_c, case.rho_d, case.rho_d_by_rho_c, stk.FP64, rho_p=case.rho,
+                      rho_c=case.rho_c)
+    chunk.set_state(*case.state())
+    chunk.upload()
+    return field, chunk
+
+
+def oracle_dual(case, direction, cd=1.0, xw=None, vw=None, T=None):
+    st = case.state()
+    tsrc = np.zeros((case.ncell, 3))
+    ts = np.zeros((6, case.np))
+    src, _ = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
+                              boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d, rho_c=case.rho_c,
+                              cd=cd, rho=case.rho, tan_d=direction[0], tan_rho=direction[1],
+                              tan_cd=direction[2], tan_source=tsrc, tan_state=ts)
+    val = 0.0
+    if T is not None:
+        val += np.sum(2.0 * (src - T) * tsrc)
+    if xw is not None:
+        val += np.sum(xw * ts[0:3])
+    if vw is not None:
+        val += np.sum(vw * ts[3:6])
+    return val
+
+
+@pytest.mark.parametrize("method", [stk.PLAIN, stk.SORTED])
+@pytest.mark.parametrize("boundary", [0, 1])
+@pytest.mark.parametrize("K", [10, 7, 40])
+def test_adjoint_vs_dual_source_misfit(ctx, boundary, method, K):
+    case = make_case(boundary=boundary)
+    T = target(case)
+    field, chunk = gpu_objects(ctx, case)
+    p = stk.params(boundary=boundary, method=method, brick=(4, 4, 4))
+    adj = Adjoint(ctx, chunk, field, case.nsub, K, p)
+    adj.forward(case.dt_fluid)
+    adj.source_misfit(T)
+    adj.backward()
+    g = adj.gradients()
+    rng = np.random.default_rng(3)
+    dd = rng.normal(size=case.np) * case.d
+    dr = rng.normal(size=case.np) * case.rho
+    dc = 0.7
+    ref = oracle_dual(case, (dd, dr, dc), T=T)
+    got = np.dot(g["d"], dd) + np.dot(g["rho"], dr) + g["cd"] * dc
+    assert abs(got - ref) <= 1e-9 * abs(ref), (got, ref)
+    # per-component: cd alone and one parcel's d alone
+    ref_cd = oracle_dual(case, (None, None, 1.0), T=T)
+    assert abs(g["cd"] - ref_cd) <= 1e-9 * abs(ref_cd)
+    e = np.zeros(case.np)
+    e[17] = 1.0
+    ref_d17 = oracle_dual(case, (e, None, 0.0), T=T)
+    assert abs(g["d"][17] - ref_d17) <= 1e-8 * abs(ref_d17) + 1e-30
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_adjoint_endpoint_loss(ctx, boundary):
+    """L = sum(wx * x_T) + sum(wv * v_T): the state cotangents propagate through."""
+    case = make_case(boundary=boundary, np_parcels=1500)
+    field, chunk = gpu_objects(ctx, case)
+    rng = np.random.default_rng(9)
+    wx = rng.normal(size=(3, case.np))
+    wv = rng.normal(size=(3, case.np))
+    adj = Adjoint(ctx, chunk, field, case.nsub, 10, stk.params(boundary=boundary))
+    adj.forward(case.dt_fluid)
+    adj.set_state_cotangent(wx, wv)
+    adj.backward()
+    g = adj.gradients(initial_state=True)
+    dd = rng.normal(size=case.np) * case.d
+    ref = oracle_dual(case, (dd, None, 1.0), xw=wx, vw=wv)
+    got = np.dot(g["d"], dd) + g["cd"]
+    assert abs(got - ref) <= 1e-9 * abs(ref), (got, ref)
+    # x passes through unchanged; backward is repeatable
+    np.testing.assert_allclose(g["x0"], wx, rtol=0, atol=0)
+    adj.backward()
+    g2 = adj.gradients()
+    np.testing.assert_array_equal(g2["d"], g["d"])
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_adjoint_vs_frozen_event_fd(ctx, boundary):
+    """The north-star bar: adjoint vs central differences (eps = 1e-6) within 1e-4."""
+    case = make_case(boundary=boundary, np_parcels=2000)
+    T = target(case)
+    field, chunk = gpu_objects(ctx, case)
+    loss, pullback = source_misfit_rrule(ctx, chunk, field, T, case.dt_fluid, case.nsub, 20,
+                                         stk.params(boundary=boundary))
+    _, gd, grho, gcd = pullback(1.0)
+    ev = np.zeros((case.np, case.nsub), dtype=np.int64)
+    st = case.state()
+    src0, _ = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
+                               boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d, rho_c=case.rho_c,
+                               rho=case.rho, events=ev, events_mode=1)
+    assert abs(loss - np.sum((src0 - T) ** 2)) <= 1e-12 * loss
+    h = 1e-6
+    rng = np.random.default_rng(4)
+    dd = rng.normal(size=case.np) * case.d
+    dr = rng.normal(size=case.np) * case.rho
+
+    def L(sign, which):
+        st = case.state()
+        d = case.d + sign * h * dd if which == "d" else case.d
+        st[0] = d.copy()
+        rho = case.rho + sign * h * dr if which == "rho" else case.rho
+        cd = 1.0 + sign * h if which == "cd" else 1.0
+        s, _ = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
+                                boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d, rho_c=case.rho_c,
+                                cd=cd, rho=rho, events=ev, events_mode=2)
+        return np.sum((s - T) ** 2)
+
+    for which, adj_val in (("d", np.dot(gd, dd)), ("rho", np.dot(grho, dr)), ("cd", gcd)):
+        fd = (L(+1, which) - L(-1, which)) / (2 * h)
+        assert abs(fd - adj_val) <= 1e-4 * abs(fd), (which, fd, adj_val)
