@@ -155,6 +155,46 @@ int st_graph_destroy(st_graph *graph);
  * events recorded on the context stream around them; -1 if not recorded. */
 int st_last_kernel_ms(st_ctx *ctx, double *ms, int32_t *launches);
 
+/* ---- batches of clouds ---------------------------------------------------------- */
+/* Extension: n independent clouds, each with its own field and mesh, advanced together
+ * by one persistent launch per pass.  The reference runs one kernel launch per chunk
+ * (asyncParallelTracking.jl:1098-1125); a batch replaces those launches.  All clouds
+ * share the precision and params (boundary, flush, block, substeps_per_pass, cd).
+ * target_parcels_per_brick > 0 picks each cloud's brick so that a brick holds about
+ * that many parcels (0: params->brick for all).  blocks_per_cu caps the persistent grid
+ * per CU (0: as many as fit).  Clouds are sorted at creation; st_batch_sort re-sorts. */
+typedef struct st_batch st_batch;
+int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *const *fields,
+                    const st_params *params, int32_t target_parcels_per_brick,
+                    int32_t blocks_per_cu, st_batch **out);
+int st_batch_destroy(st_batch *batch);
+int st_batch_sort(st_batch *batch);
+/* nsub substeps of dt_fluid / nsub for every cloud; sources accumulate per field. */
+int st_batch_evolve(st_batch *batch, double dt_fluid, int32_t nsub);
+/* One fluid step of the whole batch as a hipGraph (optionally zeroing every UTrans). */
+int st_batch_graph_capture(st_batch *batch, double dt_fluid, int32_t nsub, int32_t zero_source,
+                           st_graph **out);
+/* Work items, persistent grid size, and the brick of each cloud (3*n, caller order);
+ * any output may be NULL. */
+int st_batch_info(const st_batch *batch, int32_t *total_items, int32_t *grid,
+                  int32_t *brick_of_cloud);
+
+/* Extension: autotuner of a batch.  Times every candidate (parcels per brick in
+ * {1024 .. 16384}, block in {256, 512, 1024}, persistent workgroups per CU capped at
+ * {all, 4, 2}) over `trials` passes of one substep (or base->substeps_per_pass) and
+ * restores the clouds and sources afterwards.  The result is cached by device and
+ * workload in cache_path (JSON lines; NULL or "" disables the cache). */
+typedef struct st_tune_result {
+    int32_t target_parcels_per_brick;
+    int32_t block;
+    int32_t blocks_per_cu;
+    int32_t from_cache;  /* 1 if read from cache_path */
+    double ms_per_pass;
+} st_tune_result;
+int st_autotune_batch(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *const *fields,
+                      const st_params *base, double dt_fluid, int32_t trials, const char *cache_path,
+                      st_tune_result *out);
+
 /* ---- adjoint (FP64) -------------------------------------------------------------- */
 /* Extension: reverse-mode gradients of a loss of the deposited source (and optionally of
  * the final parcel state) with respect to the per-parcel diameter, the per-parcel density

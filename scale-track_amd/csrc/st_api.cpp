@@ -137,7 +137,14 @@ int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
     ST_HIP(launch_scan(c->counts, (int32_t)nbr, c->brick_start, c->cursor, s));
     ST_HIP(launch_scatter<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur),
                                  c->pid[1 - c->cur], c->keys, c->cursor, np, s));
-    c->cur = 1 - c->cur;
+    // Copy the sorted order back so the cloud's device pointers never change: graphs
+    // and batch tables captured earlier stay valid across re-sorts.
+    const size_t bytes = (size_t)c->n * c->es;
+    for (int f = 0; f < kNF; ++f) {
+        if (f == 7 && !c->has_rho) continue;
+        ST_HIP(hipMemcpyAsync(c->buf[c->cur][f], c->buf[1 - c->cur][f], bytes, hipMemcpyDeviceToDevice, s));
+    }
+    ST_HIP(hipMemcpyAsync(c->pid[c->cur], c->pid[1 - c->cur], (size_t)c->n * 4, hipMemcpyDeviceToDevice, s));
     c->permuted = true;
     c->sorted = true;
     for (int a = 0; a < 3; ++a) {
@@ -696,7 +703,10 @@ int st_graph_launch(st_graph *g) {
     if (!g) return fail(ST_E_INVALID, "null graph");
     ST_TRY(check_ctx(g->ctx));
     ST_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
-    if (g->owner_flush) g->field->stray_dirty = true;
+    if (g->owner_flush) {
+        if (g->field) g->field->stray_dirty = true;
+        for (st_field *f : g->more_fields) f->stray_dirty = true;
+    }
     return ST_OK;
 }
 

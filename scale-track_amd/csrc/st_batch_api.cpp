@@ -1,0 +1,304 @@
+// st_batch_api.cpp -- many independent clouds advanced by one persistent launch per pass.
+//
+// Each cloud keeps its own mesh, field and brick sort.  Work items are (cloud, brick)
+// pairs, handed out by an atomic counter to a grid sized to fill the GPU once; clouds
+// are ordered densest first so the longest items start first.  Brick dimensions are
+// chosen per cloud so a brick holds about `target_parcels_per_brick` parcels.
+#include "st_handles.h"
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+using namespace st::host;
+
+struct st_batch {
+    st_ctx *ctx = nullptr;
+    int prec = ST_FP64;
+    st_params p{};
+    int32_t target = 0;
+    int32_t blocks_per_cu = 0;
+    std::vector<st_cloud *> clouds;  // in work order (densest first)
+    std::vector<st_field *> fields;
+    std::vector<st_params> cloud_params;
+    std::vector<int> caller_index;   // slot -> position in the caller's arrays
+    void *descs = nullptr;           // device PassArgs<T>[n]
+    std::vector<unsigned char> host_descs;
+    int32_t *item0 = nullptr;        // device [n + 1]
+    unsigned int *counter = nullptr; // device
+    int32_t total_items = 0;
+    int grid = 0;
+    size_t lds = 0;
+    double dt_sub = -1.0;            // substep the device table holds
+};
+
+namespace {
+
+// Brick of about `cells` cells: powers of two, x extent largest (rows stay contiguous).
+void auto_brick(int64_t cells, const int32_t n[3], int32_t b[3]) {
+    int64_t c = 1;
+    while (c * 2 <= cells && c < 2048) c *= 2;
+    b[0] = b[1] = b[2] = 1;
+    int axis = 0;
+    while (c > 1) {
+        if (b[axis] < n[axis]) {
+            b[axis] *= 2;
+            c /= 2;
+        } else if (b[0] >= n[0] && b[1] >= n[1] && b[2] >= n[2]) {
+            break;
+        }
+        axis = (axis + 1) % 3;
+    }
+}
+
+template <class T>
+int build_table(st_batch *b, double dt_sub) {
+    const size_t n = b->clouds.size();
+    b->host_descs.assign(n * sizeof(st::PassArgs<T>), 0);
+    st::PassArgs<T> *hd = (st::PassArgs<T> *)b->host_descs.data();
+    std::vector<int32_t> item0(n + 1, 0);
+    int32_t maxcells = 1;
+    for (size_t i = 0; i < n; ++i) {
+        st_cloud *c = b->clouds[i];
+        st_field *f = b->fields[i];
+        const st_params &p = b->cloud_params[i];
+        st::PassArgs<T> a{};
+        a.s = soa<T>(c, c->cur);
+        a.U = (const T *)f->U;
+        a.src = f->src;
+        a.stray = f->stray;
+        a.brick_start = c->brick_start;
+        a.g = f->geo;
+        a.ph = make_phys<T>(c, f, dt_sub, p.cd);
+        a.st.bbox = c->bbox;
+        a.st.strays = c->strays;
+        a.nbricks = c->nbricks;
+        for (int k = 0; k < 3; ++k) {
+            a.nb[k] = c->nb[k];
+            a.bd[k] = p.brick[k];
+        }
+        a.tile_cells = p.brick[0] * p.brick[1] * p.brick[2];
+        a.nsteps = 1;
+        a.flush = p.flush;
+        a.np = (uint32_t)c->n;
+        hd[i] = a;
+        item0[i + 1] = item0[i] + c->nbricks;
+        maxcells = std::max(maxcells, a.tile_cells);
+    }
+    b->total_items = item0[n];
+    b->lds = (size_t)maxcells * 3 * (sizeof(double) + sizeof(T));
+    hipStream_t s = b->ctx->stream;
+    ST_HIP(hipMemcpyAsync(b->descs, hd, b->host_descs.size(), hipMemcpyHostToDevice, s));
+    ST_HIP(hipMemcpyAsync(b->item0, item0.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    ST_HIP(hipStreamSynchronize(s));
+    int dev = b->ctx->device;
+    int cus = 0;
+    ST_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int per = st::batch_max_blocks_per_cu<T>(b->p.boundary == ST_PERIODIC, b->p.block, b->lds);
+    if (per < 1) return fail(ST_E_RANGE, "batch kernel does not fit: block %d, %zu B LDS", b->p.block, b->lds);
+    if (b->blocks_per_cu > 0) per = std::min(per, b->blocks_per_cu);
+    b->grid = std::max(1, std::min(cus * per, b->total_items));
+    b->dt_sub = dt_sub;
+    return ST_OK;
+}
+
+int table_for(st_batch *b, double dt_sub) {
+    if (b->dt_sub == dt_sub && b->descs) return ST_OK;
+    return b->prec == ST_FP64 ? build_table<double>(b, dt_sub) : build_table<float>(b, dt_sub);
+}
+
+int sort_all(st_batch *b) {
+    for (size_t i = 0; i < b->clouds.size(); ++i) ST_TRY(sort_any(b->clouds[i], b->fields[i], b->cloud_params[i]));
+    b->dt_sub = -1.0;  // brick counts may have changed: rebuild the table
+    return ST_OK;
+}
+
+int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
+    ST_TRY(table_for(b, dt_sub));
+    hipStream_t s = b->ctx->stream;
+    for (st_cloud *c : b->clouds) ST_HIP(st::launch_reset_stats(st::Stats{c->bbox, c->strays}, s));
+    const int per = b->p.substeps_per_pass > 0 ? b->p.substeps_per_pass : nsub;
+    if (!capturing) ST_HIP(hipEventRecord(b->ctx->ev[0], s));
+    int launches = 0;
+    for (int done = 0; done < nsub; done += per) {
+        const int ns = std::min(per, nsub - done);
+        ST_HIP(st::launch_fill_u32(b->counter, 0u, 1, s));
+        if (b->prec == ST_FP64)
+            ST_HIP(st::launch_pass_batch<double>((const st::PassArgs<double> *)b->descs, b->item0,
+                                                 (int32_t)b->clouds.size(), ns, b->counter,
+                                                 b->p.boundary == ST_PERIODIC, b->p.block, b->grid, b->lds, s));
+        else
+            ST_HIP(st::launch_pass_batch<float>((const st::PassArgs<float> *)b->descs, b->item0,
+                                                (int32_t)b->clouds.size(), ns, b->counter,
+                                                b->p.boundary == ST_PERIODIC, b->p.block, b->grid, b->lds, s));
+        ++launches;
+    }
+    if (!capturing) {
+        ST_HIP(hipEventRecord(b->ctx->ev[1], s));
+        b->ctx->timed_launches = launches;
+    }
+    if (b->p.flush == ST_FLUSH_OWNER)
+        for (st_field *f : b->fields) f->stray_dirty = true;
+    return ST_OK;
+}
+
+int destroy(st_batch *b) {
+    if (!b) return ST_OK;
+    if (b->ctx) {
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+    }
+    if (b->descs) (void)hipFree(b->descs);
+    if (b->item0) (void)hipFree(b->item0);
+    if (b->counter) (void)hipFree(b->counter);
+    delete b;
+    return ST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *const *fields,
+                    const st_params *params, int32_t target_parcels_per_brick, int32_t blocks_per_cu,
+                    st_batch **out) {
+    if (!out) return fail(ST_E_INVALID, "null output handle");
+    *out = nullptr;
+    ST_TRY(check_ctx(ctx));
+    if (n < 1 || !clouds || !fields) return fail(ST_E_INVALID, "need n >= 1 clouds and fields");
+    if (target_parcels_per_brick < 0 || blocks_per_cu < 0)
+        return fail(ST_E_INVALID, "target_parcels_per_brick and blocks_per_cu must be >= 0");
+    st_params p;
+    if (params)
+        p = *params;
+    else
+        st_params_default(&p);
+    p.method = ST_METHOD_SORTED;
+    ST_TRY(validate_params(p));
+    std::vector<int> idx(n);
+    std::vector<double> density(n);
+    for (int i = 0; i < n; ++i) {
+        st_params tmp;
+        ST_TRY(prepare(ctx, clouds[i], fields[i], 1.0, 1, &p, tmp));
+        if (clouds[i]->prec != clouds[0]->prec) return fail(ST_E_INVALID, "all clouds of a batch share one precision");
+        for (int j = 0; j < i; ++j)
+            if (clouds[j] == clouds[i] || fields[j] == fields[i])
+                return fail(ST_E_INVALID, "a cloud or field appears twice in the batch");
+        idx[i] = i;
+        density[i] = (double)clouds[i]->n / (double)fields[i]->ncell;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return density[a] > density[c]; });
+    st_batch *b = new (std::nothrow) st_batch;
+    if (!b) return fail(ST_E_NOMEM, "host allocation failed");
+    b->ctx = ctx;
+    b->prec = clouds[0]->prec;
+    b->p = p;
+    b->target = target_parcels_per_brick;
+    b->blocks_per_cu = blocks_per_cu;
+    for (int k = 0; k < n; ++k) {
+        const int i = idx[k];
+        b->clouds.push_back(clouds[i]);
+        b->fields.push_back(fields[i]);
+        b->caller_index.push_back(i);
+        st_params pc = p;
+        if (target_parcels_per_brick > 0) {
+            const double want = target_parcels_per_brick / std::max(density[i], 1e-9);
+            auto_brick((int64_t)std::max(1.0, want), fields[i]->mesh.n, pc.brick);
+        }
+        b->cloud_params.push_back(pc);
+    }
+    hipError_t e = hipMalloc(&b->descs, (size_t)n * std::max(sizeof(st::PassArgs<double>), sizeof(st::PassArgs<float>)));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->item0, (size_t)(n + 1) * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->counter, sizeof(unsigned int));
+    if (e != hipSuccess) {
+        destroy(b);
+        return fail(ST_E_NOMEM, "batch allocation: %s", hipGetErrorString(e));
+    }
+    int r = sort_all(b);
+    if (r == ST_OK) r = table_for(b, 0.0);  // sizes the grid; the first evolve sets dt
+    if (r != ST_OK) {
+        destroy(b);
+        return r;
+    }
+    *out = b;
+    return ST_OK;
+}
+
+int st_batch_destroy(st_batch *b) { return destroy(b); }
+
+int st_batch_sort(st_batch *b) {
+    if (!b) return fail(ST_E_INVALID, "null batch");
+    ST_TRY(check_ctx(b->ctx));
+    return sort_all(b);
+}
+
+int st_batch_evolve(st_batch *b, double dt_fluid, int32_t nsub) {
+    if (!b) return fail(ST_E_INVALID, "null batch");
+    ST_TRY(check_ctx(b->ctx));
+    if (!std::isfinite(dt_fluid) || !(dt_fluid > 0)) return fail(ST_E_INVALID, "dt_fluid must be finite and > 0");
+    if (nsub < 1) return fail(ST_E_INVALID, "nsub must be >= 1");
+    return queue_passes(b, substep_dt(b->prec, dt_fluid, nsub), nsub, false);
+}
+
+int st_batch_graph_capture(st_batch *b, double dt_fluid, int32_t nsub, int32_t zero_source, st_graph **out) {
+    if (!out) return fail(ST_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (!b) return fail(ST_E_INVALID, "null batch");
+    ST_TRY(check_ctx(b->ctx));
+    if (!std::isfinite(dt_fluid) || !(dt_fluid > 0)) return fail(ST_E_INVALID, "dt_fluid must be finite and > 0");
+    if (nsub < 1) return fail(ST_E_INVALID, "nsub must be >= 1");
+    const double dt_sub = substep_dt(b->prec, dt_fluid, nsub);
+    ST_TRY(table_for(b, dt_sub));  // table upload happens before the capture
+    st_graph *g = new (std::nothrow) st_graph;
+    if (!g) return fail(ST_E_NOMEM, "host allocation failed");
+    g->ctx = b->ctx;
+    g->more_fields = b->fields;
+    g->owner_flush = b->p.flush == ST_FLUSH_OWNER;
+    hipStream_t s = b->ctx->stream;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(ST_E_DEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    int r = ST_OK;
+    if (zero_source) {
+        for (st_field *f : b->fields) {
+            e = st::launch_fill_f64(f->src, 0.0, (size_t)f->ncell * 3, s);
+            if (e == hipSuccess && g->owner_flush) e = st::launch_fill_f64(f->stray, 0.0, (size_t)f->ncell * 3, s);
+            if (e != hipSuccess) {
+                r = fail(ST_E_DEVICE, "capture fill: %s", hipGetErrorString(e));
+                break;
+            }
+        }
+    }
+    if (r == ST_OK) r = queue_passes(b, dt_sub, nsub, true);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(s, &graph);
+    if (r != ST_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        delete g;
+        return r != ST_OK ? r : fail(ST_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        delete g;
+        return fail(ST_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    g->graph = graph;
+    *out = g;
+    return ST_OK;
+}
+
+int st_batch_info(const st_batch *b, int32_t *total_items, int32_t *grid, int32_t *brick_of_cloud) {
+    if (!b) return fail(ST_E_INVALID, "null batch");
+    if (total_items) *total_items = b->total_items;
+    if (grid) *grid = b->grid;
+    if (brick_of_cloud)  // in the caller's cloud order
+        for (size_t i = 0; i < b->clouds.size(); ++i)
+            for (int k = 0; k < 3; ++k) brick_of_cloud[3 * b->caller_index[i] + k] = b->cloud_params[i].brick[k];
+    return ST_OK;
+}
+
+}  // extern "C"

@@ -5,6 +5,7 @@
 #include "st_internal.h"
 
 #include <string>
+#include <vector>
 
 namespace st {
 namespace host {
@@ -91,6 +92,7 @@ struct st_field {
 struct st_graph {
     st_ctx *ctx = nullptr;
     st_field *field = nullptr;
+    std::vector<st_field *> more_fields;  // batch graphs: every field the graph deposits into
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool owner_flush = false;

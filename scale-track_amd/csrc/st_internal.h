@@ -88,6 +88,12 @@ template <class T>
 hipError_t launch_pass_sorted(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
 template <class T>
 hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
+template <class T>
+hipError_t launch_pass_batch(const PassArgs<T> *descs, const int32_t *item0, int32_t ncloud, int32_t nsteps,
+                             unsigned int *counter, int periodic, int block, int grid, size_t lds,
+                             hipStream_t s);
+template <class T>
+int batch_max_blocks_per_cu(int periodic, int block, size_t lds);
 hipError_t launch_reset_stats(Stats st, hipStream_t s);
 template <class T>
 hipError_t launch_brick_keys(const Soa<T> &s, uint32_t np, Geo g, const int32_t nb[3],

@@ -236,10 +236,11 @@ __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
     flush_stats(box, 0u, a.st);
 }
 
+// One brick of a sorted cloud: stage its velocities in LDS, step its parcels, add its
+// source sums to HBM once per cell.  Every thread of the workgroup must call it.
 template <class T, bool PERIODIC>
-__global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
-    extern __shared__ double lds[];
-    const int32_t brick = xcd_remap(blockIdx.x, a.nbricks);
+__device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t brick, int32_t nsteps,
+                                              double *lds) {
     const int32_t bi = brick % a.nb[0];
     const int32_t bj = (brick / a.nb[0]) % a.nb[1];
     const int32_t bk = brick / (a.nb[0] * a.nb[1]);
@@ -290,7 +291,7 @@ __global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
     if (p < p1) load_parcel(a.s, (uint32_t)p, a.ph.rho, cur);
     for (; p < p1; p += blockDim.x) {
         const int32_t q = p + (int32_t)blockDim.x;
-        step_parcel<T, PERIODIC>(a.g, a.ph, a.s, cur, (uint32_t)p, a.nsteps, env, box, [&] {
+        step_parcel<T, PERIODIC>(a.g, a.ph, a.s, cur, (uint32_t)p, nsteps, env, box, [&] {
             if (q < p1) load_parcel(a.s, (uint32_t)q, a.ph.rho, nxt);
         });
         cur = nxt;
@@ -322,6 +323,68 @@ __global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
     }
     flush_stats(box, env.strays, a.st);
 }
+
+template <class T, bool PERIODIC>
+__global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
+    extern __shared__ double lds[];
+    process_brick<T, PERIODIC>(a, xcd_remap(blockIdx.x, a.nbricks), a.nsteps, lds);
+}
+
+// Many clouds in one persistent launch: each workgroup takes (cloud, brick) work items
+// from a global counter until none are left.  item0[c] is the first item of cloud c
+// (clouds ordered densest first, so the longest items are handed out first); every
+// workgroup leaves once the counter passes item0[ncloud].
+template <class T, bool PERIODIC>
+__global__ void __launch_bounds__(1024) k_pass_batch(const PassArgs<T> *__restrict__ descs,
+                                                     const int32_t *__restrict__ item0, int32_t ncloud,
+                                                     int32_t nsteps, unsigned int *counter) {
+    extern __shared__ double lds[];
+    __shared__ int32_t s_item;
+    const int32_t total = item0[ncloud];
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int32_t)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= total) break;
+        int32_t lo = 0, hi = ncloud - 1;  // last cloud with item0 <= item
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (item0[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int32_t c = __builtin_amdgcn_readfirstlane(lo);
+        const PassArgs<T> a = descs[c];
+        process_brick<T, PERIODIC>(a, item - item0[c], nsteps, lds);
+        __syncthreads();  // s_item is rewritten only after every thread has read it
+    }
+}
+
+template <class T>
+hipError_t launch_pass_batch(const PassArgs<T> *descs, const int32_t *item0, int32_t ncloud, int32_t nsteps,
+                             unsigned int *counter, int periodic, int block, int grid, size_t lds,
+                             hipStream_t s) {
+    if (periodic)
+        k_pass_batch<T, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, counter);
+    else
+        k_pass_batch<T, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, counter);
+    return hipGetLastError();
+}
+template hipError_t launch_pass_batch<float>(const PassArgs<float> *, const int32_t *, int32_t, int32_t,
+                                             unsigned int *, int, int, int, size_t, hipStream_t);
+template hipError_t launch_pass_batch<double>(const PassArgs<double> *, const int32_t *, int32_t, int32_t,
+                                              unsigned int *, int, int, int, size_t, hipStream_t);
+
+template <class T>
+int batch_max_blocks_per_cu(int periodic, int block, size_t lds) {
+    int n = 0;
+    hipError_t e = periodic ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_batch<T, true>, block, lds)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_batch<T, false>, block, lds);
+    return e == hipSuccess ? n : 0;
+}
+template int batch_max_blocks_per_cu<float>(int, int, size_t);
+template int batch_max_blocks_per_cu<double>(int, int, size_t);
 
 template <class T>
 hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s) {

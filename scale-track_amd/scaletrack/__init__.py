@@ -5,8 +5,8 @@ Importing this package loads libscaletrack.so and fails loudly if it is not buil
 from . import _lib  # noqa: F401  (loads the library)
 from .tracking import (  # noqa: F401
     FP32, FP64, FLUSH_ATOMIC, FLUSH_OWNER, PERIODIC, PLAIN, SORTED, WALL, Chunk, Context, Graph,
-    Mesh, Time, TwoWayEulerian, construct_mesh, device_count, evolve, evolve_device,
-    increment_time, params)
+    Batch, Mesh, Time, TwoWayEulerian, construct_mesh, device_count, evolve, evolve_device,
+    autotune_batch, increment_time, params)
 from ._lib import ScaleTrackError  # noqa: F401
 
 __version__ = "0.1.0"

@@ -53,6 +53,16 @@ class st_params(C.Structure):
     ]
 
 
+class st_tune_result(C.Structure):
+    _fields_ = [
+        ("target_parcels_per_brick", C.c_int32),
+        ("block", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+        ("from_cache", C.c_int32),
+        ("ms_per_pass", C.c_double),
+    ]
+
+
 class ScaleTrackError(RuntimeError):
     """A nonzero status from the C-ABI, with the library's message."""
 
@@ -94,6 +104,14 @@ SIGNATURES = {
     "st_graph_destroy": (C.c_int, [_vp]),
     "st_last_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     "st_diag_copy_bandwidth": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "st_batch_create": (C.c_int, [_vp, C.c_int32, _pp, _pp, C.POINTER(st_params), C.c_int32, C.c_int32, _pp]),
+    "st_batch_destroy": (C.c_int, [_vp]),
+    "st_batch_sort": (C.c_int, [_vp]),
+    "st_batch_evolve": (C.c_int, [_vp, C.c_double, C.c_int32]),
+    "st_batch_graph_capture": (C.c_int, [_vp, C.c_double, C.c_int32, C.c_int32, _pp]),
+    "st_batch_info": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "st_autotune_batch": (C.c_int, [_vp, C.c_int32, _pp, _pp, C.POINTER(st_params), C.c_double, C.c_int32,
+                                    C.c_char_p, C.POINTER(st_tune_result)]),
     "st_adjoint_create": (C.c_int, [_vp, _vp, _vp, C.c_int32, C.c_int32, C.POINTER(st_params), _pp]),
     "st_adjoint_destroy": (C.c_int, [_vp]),
     "st_adjoint_forward": (C.c_int, [_vp, C.c_double]),

@@ -278,3 +278,71 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Independent clouds advanced together by one persistent launch per pass (st_batch).
+    `target_parcels_per_brick` > 0 sizes each cloud's brick to hold about that many
+    parcels; `blocks_per_cu` caps the persistent grid (0: as many as fit)."""
+
+    def __init__(self, ctx, chunks, fields, p=None, target_parcels_per_brick=4096, blocks_per_cu=0):
+        n = len(chunks)
+        assert n == len(fields) and n > 0
+        self.ctx, self.chunks, self.fields = ctx, list(chunks), list(fields)
+        arr_c = (C.c_void_p * n)(*[c.handle.value for c in chunks])
+        arr_f = (C.c_void_p * n)(*[f.handle.value for f in fields])
+        h = C.c_void_p()
+        check(lib.st_batch_create(ctx.handle, n, C.cast(arr_c, C.POINTER(C.c_void_p)),
+                                  C.cast(arr_f, C.POINTER(C.c_void_p)),
+                                  C.byref(p) if p is not None else None, int(target_parcels_per_brick),
+                                  int(blocks_per_cu), C.byref(h)))
+        self.handle = h
+
+    def evolve(self, dt_fluid, nsub=10):
+        check(lib.st_batch_evolve(self.handle, float(dt_fluid), int(nsub)))
+
+    def sort(self):
+        check(lib.st_batch_sort(self.handle))
+
+    def info(self):
+        n = len(self.chunks)
+        items, grid = C.c_int32(), C.c_int32()
+        bricks = (C.c_int32 * (3 * n))()
+        check(lib.st_batch_info(self.handle, C.byref(items), C.byref(grid), bricks))
+        return {"items": items.value, "grid": grid.value,
+                "bricks": [tuple(bricks[3 * i:3 * i + 3]) for i in range(n)]}
+
+    def graph(self, dt_fluid, nsub=10, zero_source=True):
+        g = Graph.__new__(Graph)
+        h = C.c_void_p()
+        check(lib.st_batch_graph_capture(self.handle, float(dt_fluid), int(nsub), int(bool(zero_source)),
+                                         C.byref(h)))
+        g.handle = h
+        return g
+
+    def close(self):
+        if self.handle:
+            lib.st_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune_batch(ctx, chunks, fields, p=None, dt_fluid=1e-3, trials=3, cache_path=None):
+    """st_autotune_batch: the fastest (target parcels per brick, block, workgroups per CU)
+    for these clouds, timed on a snapshot that is restored afterwards; cached per device
+    and workload in `cache_path` (JSON lines) when given."""
+    n = len(chunks)
+    arr_c = (C.c_void_p * n)(*[c.handle.value for c in chunks])
+    arr_f = (C.c_void_p * n)(*[f.handle.value for f in fields])
+    r = _lib.st_tune_result()
+    check(lib.st_autotune_batch(ctx.handle, n, C.cast(arr_c, C.POINTER(C.c_void_p)),
+                                C.cast(arr_f, C.POINTER(C.c_void_p)), C.byref(p) if p is not None else None,
+                                float(dt_fluid), int(trials),
+                                None if cache_path is None else str(cache_path).encode(), C.byref(r)))
+    return {"target_parcels_per_brick": r.target_parcels_per_brick, "block": r.block,
+            "blocks_per_cu": r.blocks_per_cu, "ms_per_pass": r.ms_per_pass, "from_cache": bool(r.from_cache)}
