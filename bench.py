@@ -38,13 +38,16 @@ def parse():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--method", choices=["sorted", "plain"], default="sorted")
     ap.add_argument("--substeps-per-pass", type=int, default=1)
-    ap.add_argument("--brick", type=str, default="8,8,8")
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--brick", type=str, default="16,8,4")
+    ap.add_argument("--block", type=int, default=512)
     ap.add_argument("--flush", choices=["owner", "atomic"], default="owner")
     ap.add_argument("--only", action="store_true",
                     help="time only the headline configuration (for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-parcels", type=int, default=1 << 22)
+    ap.add_argument("--tune-cache", default=None, help="autotuner cache file (JSON lines)")
+    ap.add_argument("--workload", choices=["forward", "batch", "adjoint"], default="forward",
+                    help="with --only: which workload to time (profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the streaming copy kernel over BYTES (traffic-counter calibration)")
     return ap.parse_args()
@@ -96,6 +99,112 @@ def run_variant(stk, ctx, case, precision, p, steps, warmup, keep_state=False):
     return out
 
 
+def batch_workload(stk, inputs, ctx, nclouds=64, steps=5, warmup=1, seed=19891, tune_cache=None):
+    """64 independent periodic clouds, parcel counts log-uniform in [1e3, 2e6], meshes
+    n^3 with n uniform in {32..96}, Taylor-Green with seeded random phases, FP32 state;
+    one persistent launch per substep, the fluid step captured as a hipGraph, brick and
+    block sizes from the autotuner."""
+    u = inputs.uniform(seed, 100, 5 * nclouds).reshape(nclouds, 5)
+    counts = np.rint(np.exp(np.log(1e3) + u[:, 0] * (np.log(2e6) - np.log(1e3)))).astype(np.int64)
+    cells = (32 + np.floor(u[:, 1] * 65)).astype(np.int64)
+    chunks, fields, bytes_pass = [], [], 0
+    for i in range(nclouds):
+        c = inputs.taylor_green_box(np_parcels=int(counts[i]), ncell=int(cells[i]), dtype=np.float32,
+                                    boundary=stk.PERIODIC, seed=seed + 1000 + i)
+        c.U = inputs.taylor_green(c.n, c.origin, c.ending, phase=tuple(2 * np.pi * u[i, 2:5]), dtype=np.float32)
+        field = stk.TwoWayEulerian(ctx, stk.Mesh(c.n, c.origin, c.ending), stk.FP32)
+        field.upload_U(c.U)
+        chunk = stk.Chunk(ctx, c.np, c.mu_c, c.rho_d, c.rho_d_by_rho_c, stk.FP32)
+        chunk.set_state(*c.state())
+        chunk.upload()
+        chunks.append(chunk)
+        fields.append(field)
+        bytes_pass += bytes_per_pass(c.np, c.ncell, True)
+    p = stk.params(boundary=stk.PERIODIC, substeps_per_pass=1)
+    t0 = time.perf_counter()
+    best = stk.autotune_batch(ctx, chunks, fields, p, 1e-3, trials=2, cache_path=tune_cache)
+    t_tune = time.perf_counter() - t0
+    p.block = best["block"]
+    batch = stk.Batch(ctx, chunks, fields, p, target_parcels_per_brick=best["target_parcels_per_brick"],
+                      blocks_per_cu=best["blocks_per_cu"])
+    graph = batch.graph(1e-3, 10, zero_source=True)
+    for _ in range(warmup):
+        graph.launch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.launch()
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    # kernel time per pass from an eager step (events around the passes)
+    for f in fields:
+        f.zero_source()
+    batch.evolve(1e-3, 10)
+    kms, _ = ctx.last_kernel_ms()
+    total = int(counts.sum())
+    info = batch.info()
+    graph.close()
+    batch.close()
+    for c in chunks:
+        c.close()
+    for f in fields:
+        f.close()
+    return {"clouds": nclouds, "parcels": total, "min_parcels": int(counts.min()), "max_parcels": int(counts.max()),
+            "mesh_n_min": int(cells.min()), "mesh_n_max": int(cells.max()),
+            "updates_per_s": total * 10 * steps / el, "ms_per_step": 1e3 * el / steps,
+            "kernel_ms_per_pass": kms, "roofline_frac": bytes_pass / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "bytes_per_pass": bytes_pass, "work_items": info["items"], "grid": info["grid"],
+            "tuned": best, "tune_s": t_tune, "dtype": "f32 state, f64 face test and source"}
+
+
+def adjoint_workload(stk, inputs, ctx, np_parcels=1 << 20, ncell=64, nsub=200, K=20, seed=19891):
+    """Forward + adjoint: frozen divergence-free field of 8 seeded Fourier modes, per-
+    parcel d and rho, nsub substeps of 1e-4 s with a checkpoint every K; loss = L2 misfit
+    of the deposited source against a seeded target; gradients for d, rho and cd."""
+    from scaletrack.adjoint import Adjoint
+
+    n = (ncell,) * 3
+    o, e = (0.0,) * 3, (1.0,) * 3
+    U = inputs.fourier_field(n, o, e, modes=8, seed=seed)
+    x, y, z = inputs.positions(seed, np_parcels, o, e)
+    d = inputs.lognormal_diameters(seed, np_parcels)
+    rho = 1000.0 * (1.0 + 0.5 * inputs.uniform(seed, 50, np_parcels))
+    zero = np.zeros(np_parcels)
+    field = stk.TwoWayEulerian(ctx, stk.Mesh(n, o, e), stk.FP64)
+    field.upload_U(U)
+    chunk = stk.Chunk(ctx, np_parcels, 1e-3, 1000.0, 1.0, stk.FP64, rho_p=rho, rho_c=1000.0)
+    chunk.set_state(d, x, y, z, zero, zero, zero)
+    chunk.upload()
+    T = 1e-12 * inputs.normal(seed, 60, ncell ** 3 * 3).reshape(-1, 3)
+    p = stk.params(boundary=stk.PERIODIC)
+    adj = Adjoint(ctx, chunk, field, nsub, K, p)
+    dt_fluid = nsub * 1e-4
+
+    def run():
+        chunk.set_state(d, x, y, z, zero, zero, zero)
+        chunk.upload()
+        t0 = time.perf_counter()
+        adj.forward(dt_fluid)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        loss = adj.source_misfit(T)
+        adj.backward()
+        g = adj.gradients()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, loss, g
+
+    run()  # warm-up (sort, code load)
+    tf, tb, loss, g = run()
+    bms, segs = ctx.last_kernel_ms()
+    adj.close()
+    chunk.close()
+    field.close()
+    return {"parcels": np_parcels, "cells": list(n), "substeps": nsub, "checkpoint_every": K,
+            "forward_s": tf, "adjoint_s": tb, "fwd_adjoint_updates_per_s": np_parcels * nsub / (tf + tb),
+            "forward_updates_per_s": np_parcels * nsub / tf, "reverse_kernel_ms_per_segment": bms,
+            "loss": loss, "grad_cd": g["cd"], "dtype": "f64"}
+
+
 def cpu_baseline(case, nparcels):
     """The oracle (C++ FP64 restatement, test infrastructure) on the first `nparcels`
     parcels of the same workload for one fluid step (nsub = 10 substeps), serial and
@@ -136,6 +245,13 @@ def main():
     if args.calibrate_copy:
         gbs = ctx.copy_bandwidth(args.calibrate_copy, reps=args.steps)
         print(json.dumps({"copy_bytes": args.calibrate_copy, "reps": args.steps, "GBs": gbs}))
+        return
+    if args.only and args.workload == "batch":
+        print(json.dumps(batch_workload(stk, inputs, ctx, steps=args.steps, warmup=args.warmup,
+                                        tune_cache=args.tune_cache)))
+        return
+    if args.only and args.workload == "adjoint":
+        print(json.dumps(adjoint_workload(stk, inputs, ctx)))
         return
     copy_gbs = ctx.copy_bandwidth(1 << 30, reps=10)
     t0 = time.perf_counter()
@@ -216,6 +332,12 @@ def main():
         line["variants"]["fp32_sorted_fused10"]["roofline_frac"] = (
             bytes_per_pass(case.np, nc, True) / (fused32["kernel_ms_per_pass"] * 1e-3) / 1e9 / PEAK_HBM_GBS)
         line["speedup_vs_fp64_plain"] = head["updates_per_s"] / plain64["updates_per_s"]
+    if not args.only:
+        line["batch"] = batch_workload(stk, inputs, ctx, tune_cache=args.tune_cache)
+        print(f"# batch: {json.dumps(line['batch'])}", file=sys.stderr, flush=True)
+        line["adjoint"] = adjoint_workload(stk, inputs, ctx)
+        line["adjoint_16M"] = adjoint_workload(stk, inputs, ctx, np_parcels=1 << 24, ncell=128, nsub=50, K=10)
+        print(f"# adjoint: {json.dumps(line['adjoint'])}", file=sys.stderr, flush=True)
     if not args.no_cpu and not args.only:
         omp, serial = cpu_baseline(case, min(args.cpu_parcels, case.np))
         line["cpu_baseline"] = omp

@@ -78,8 +78,8 @@ typedef struct st_params {
     int32_t flush;             /* st_flush, default ST_FLUSH_OWNER */
     int32_t substeps_per_pass; /* substeps fused in registers per launch; 0 = all nsub */
     double cd;                 /* drag-coefficient multiplier, default 1 (reference drag) */
-    int32_t brick[3];          /* cells per brick (sorted method); default 8,8,8 */
-    int32_t block;             /* threads per workgroup; default 256 */
+    int32_t brick[3];          /* cells per brick (sorted method); default 16,8,4 */
+    int32_t block;             /* threads per workgroup; default 512 */
     double resort_fraction;    /* re-sort when the share of parcels outside their brick in
                                   the last pass exceeds this; default 0.08 */
 } st_params;

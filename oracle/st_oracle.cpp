@@ -149,7 +149,7 @@ using DW = typename Wide<S>::type;
 template <class S>
 void evolve_particle(const st_oracle_args &a, const MeshD &m, const Consts<S> &k, S dia, S rho,
                      DW<S> rdr, Vec3<S> &pos, Vec3<S> &vel, const typename Store<S>::type *Uc,
-                     int nsteps, DW<S> *src, double bb[6], int64_t *ev) {
+                     int nsteps, DW<S> *src, int64_t *ev) {
     using D = DW<S>;
     const bool periodic = a.boundary == ST_O_PERIODIC;
     const int evmode = ev ? a.events_mode : 0;
@@ -255,9 +255,13 @@ void evolve_particle(const st_oracle_args &a, const MeshD &m, const Consts<S> &k
     }
     /* :1361 */
     for (int c = 0; c < 3; ++c) src[3 * posI + c] = src[3 * posI + c] + mByRho * (s0[c] - widen(velNew.c[c]));
-    /* :1362 bounding box of the new positions */
+}
+
+/* :1362 bounding box of the positions a call ends with (after all its passes) */
+template <class S>
+void bbox_add(double bb[6], const Vec3<S> &pos) {
     for (int c = 0; c < 3; ++c) {
-        const double p = val(posNew.c[c]);
+        const double p = val(pos.c[c]);
         bb[c] = std::min(bb[c], p);
         bb[3 + c] = std::max(bb[3 + c], p);
     }
@@ -302,8 +306,9 @@ void run_parcel(st_oracle_args *a, const MeshD &m, const Consts<S> &k, int64_t i
     for (int done = 0; done < a->nsub; done += per) {
         const int ns = std::min(per, a->nsub - done);
         int64_t *ev = a->events ? a->events + i * (int64_t)a->nsub + done : nullptr;
-        evolve_particle<S>(*a, m, k, dia, rho, rdr, pos, vel, (const S *)a->U, ns, src, bb, ev);
+        evolve_particle<S>(*a, m, k, dia, rho, rdr, pos, vel, (const S *)a->U, ns, src, ev);
     }
+    bbox_add(bb, pos);
     /* :1355-1360 */
     X[i] = pos.c[0];
     Y[i] = pos.c[1];
@@ -391,8 +396,9 @@ int evolve_dual(st_oracle_args *a) {
         for (int done = 0; done < a->nsub; done += per) {
             const int ns = std::min(per, a->nsub - done);
             int64_t *ev = a->events ? a->events + i * (int64_t)a->nsub + done : nullptr;
-            evolve_particle<Dual>(*a, m, k, dia, rho, rdr, pos, vel, (const double *)a->U, ns, src.data(), bb, ev);
+            evolve_particle<Dual>(*a, m, k, dia, rho, rdr, pos, vel, (const double *)a->U, ns, src.data(), ev);
         }
+        bbox_add(bb, pos);
         X[i] = pos.c[0].v;
         Y[i] = pos.c[1].v;
         Z[i] = pos.c[2].v;

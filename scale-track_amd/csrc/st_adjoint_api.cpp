@@ -38,13 +38,12 @@ namespace {
 
 int destroy(st_adjoint *a) {
     if (!a) return ST_OK;
-    if (a->ctx) {
-        (void)hipSetDevice(a->ctx->device);
-        (void)hipStreamSynchronize(a->ctx->stream);
-    }
+    quiesce_ctx(a->ctx);
     for (double *q : {a->ck, a->xb, a->vb, a->vseed, a->fbar, a->mbar, a->G, a->tmp, a->scal})
         if (q) (void)hipFree(q);
+    st_ctx *ctx = a->ctx;
     delete a;
+    release_ctx(ctx);
     return ST_OK;
 }
 
@@ -69,6 +68,7 @@ int st_adjoint_create(st_ctx *ctx, st_cloud *c, st_field *f, int32_t nsub, int32
     st_adjoint *a = new (std::nothrow) st_adjoint;
     if (!a) return fail(ST_E_NOMEM, "host allocation failed");
     a->ctx = ctx;
+    retain_ctx(ctx);
     a->cloud = c;
     a->field = f;
     a->p = p;

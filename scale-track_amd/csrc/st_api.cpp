@@ -33,8 +33,33 @@ const char *last_error_text() { return g_err.c_str(); }
 
 int check_ctx(st_ctx *ctx) {
     if (!ctx) return fail(ST_E_INVALID, "null context");
+    if (ctx->closed) return fail(ST_E_STATE, "context was destroyed");
     ST_HIP(hipSetDevice(ctx->device));
     return ST_OK;
+}
+
+void retain_ctx(st_ctx *ctx) {
+    if (ctx) ++ctx->refs;
+}
+
+static void free_ctx(st_ctx *ctx) {
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev[0]) (void)hipEventDestroy(ctx->ev[0]);
+    if (ctx->ev[1]) (void)hipEventDestroy(ctx->ev[1]);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void release_ctx(st_ctx *ctx) {
+    if (!ctx) return;
+    if (--ctx->refs <= 0 && ctx->closed) free_ctx(ctx);
+}
+
+void quiesce_ctx(st_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
 }
 
 bool finite_all(const void *p, int64_t n, size_t es) {
@@ -197,8 +222,7 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     a.brick_start = c->brick_start;
     a.g = f->geo;
     a.ph = make_phys<T>(c, f, dt_sub, p.cd);
-    a.st.bbox = c->bbox;
-    a.st.strays = c->strays;
+    a.part = nullptr;
     a.nbricks = c->nbricks;
     for (int k = 0; k < 3; ++k) {
         a.nb[k] = c->nb[k];
@@ -210,11 +234,14 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     const int per = p.substeps_per_pass > 0 ? p.substeps_per_pass : nsub;
     const int periodic = p.boundary == ST_PERIODIC;
 
-    ST_HIP(launch_reset_stats(a.st, s));
+    // Statistics (bounding box, strays) are taken in the last pass only.
+    const int32_t nslots = p.method == ST_METHOD_SORTED ? c->nbricks : (int32_t)((c->n + p.block - 1) / p.block);
+    ST_TRY(ensure_part(c, nslots));
     if (!capturing) ST_HIP(hipEventRecord(ctx->ev[0], s));
     int launches = 0;
     for (int done = 0; done < nsub; done += per) {
         a.nsteps = per < nsub - done ? per : nsub - done;
+        a.part = done + a.nsteps >= nsub ? c->part : nullptr;
         if (p.method == ST_METHOD_SORTED)
             ST_HIP(launch_pass_sorted<T>(a, periodic, p.block, s));
         else
@@ -225,12 +252,13 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
         ST_HIP(hipEventRecord(ctx->ev[1], s));
         ctx->timed_launches = launches;
     }
+    ST_HIP(launch_reduce_stats(StatsTarget{c->part, nslots, c->bbox, c->strays}, s));
     if (p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER) f->stray_dirty = true;
     if (p.method == ST_METHOD_SORTED && !capturing) {
         ST_HIP(hipMemcpyAsync(c->strays_host, c->strays, 4, hipMemcpyDeviceToHost, s));
         ST_HIP(hipEventRecord(c->strays_ev, s));
         c->strays_pending = true;
-        c->strays_den = c->n * (int64_t)launches;
+        c->strays_den = c->n;
     }
     return ST_OK;
 }
@@ -268,6 +296,16 @@ int maybe_sort(st_cloud *c, st_field *f, const st_params &p) {
 
 int sort_any(st_cloud *c, st_field *f, const st_params &p) {
     return c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p);
+}
+
+int ensure_part(st_cloud *c, int32_t nslots) {
+    if (nslots <= c->part_cap) return ST_OK;
+    if (c->part) ST_HIP(hipFree(c->part));
+    c->part = nullptr;
+    c->part_cap = 0;
+    ST_HIP(hipMalloc((void **)&c->part, (size_t)nslots * kStatSlot * sizeof(double)));
+    c->part_cap = nslots;
+    return ST_OK;
 }
 
 int merge_source(st_field *f) {
@@ -313,10 +351,10 @@ void st_params_default(st_params *p) {
     p->flush = ST_FLUSH_OWNER;
     p->substeps_per_pass = 0;
     p->cd = 1.0;
-    p->brick[0] = 8;
+    p->brick[0] = 16;
     p->brick[1] = 8;
-    p->brick[2] = 8;
-    p->block = 256;
+    p->brick[2] = 4;
+    p->block = 512;
     p->resort_fraction = 0.08;
 }
 
@@ -355,13 +393,10 @@ int st_ctx_create(int32_t device, st_ctx **out) {
 }
 
 int st_ctx_destroy(st_ctx *ctx) {
-    if (!ctx) return ST_OK;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    if (ctx->ev[0]) hipEventDestroy(ctx->ev[0]);
-    if (ctx->ev[1]) hipEventDestroy(ctx->ev[1]);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
-    delete ctx;
+    if (!ctx || ctx->closed) return ST_OK;
+    quiesce_ctx(ctx);
+    ctx->closed = true;
+    if (ctx->refs <= 0) free_ctx(ctx);
     return ST_OK;
 }
 
@@ -388,6 +423,7 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
     st_cloud *c = new (std::nothrow) st_cloud;
     if (!c) return fail(ST_E_NOMEM, "host allocation failed");
     c->ctx = ctx;
+    retain_ctx(ctx);
     c->prec = precision;
     c->es = precision == ST_FP64 ? 8 : 4;
     c->n = n;
@@ -398,7 +434,7 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
     c->has_rho = per_parcel_rho ? 1 : 0;
     int r = alloc_set(c, 0);
     hipError_t e = hipSuccess;
-    if (r == ST_OK) e = hipMalloc((void **)&c->bbox, 6 * sizeof(unsigned long long));
+    if (r == ST_OK) e = hipMalloc((void **)&c->bbox, 6 * sizeof(double));
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->strays, 4);
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->bad, 4);
     if (r == ST_OK && e == hipSuccess) e = hipHostMalloc((void **)&c->strays_host, 4, hipHostMallocDefault);
@@ -415,10 +451,7 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
 
 int st_cloud_destroy(st_cloud *c) {
     if (!c) return ST_OK;
-    if (c->ctx) {
-        hipSetDevice(c->ctx->device);
-        hipStreamSynchronize(c->ctx->stream);
-    }
+    quiesce_ctx(c->ctx);
     for (int w = 0; w < 2; ++w) {
         for (int f = 0; f < kNF; ++f)
             if (c->buf[w][f]) hipFree(c->buf[w][f]);
@@ -429,11 +462,14 @@ int st_cloud_destroy(st_cloud *c) {
     if (c->cursor) hipFree(c->cursor);
     if (c->brick_start) hipFree(c->brick_start);
     if (c->bbox) hipFree(c->bbox);
+    if (c->part) hipFree(c->part);
     if (c->strays) hipFree(c->strays);
     if (c->bad) hipFree(c->bad);
     if (c->strays_host) hipHostFree(c->strays_host);
     if (c->strays_ev) hipEventDestroy(c->strays_ev);
+    st_ctx *ctx = c->ctx;
     delete c;
+    release_ctx(ctx);
     return ST_OK;
 }
 
@@ -515,6 +551,7 @@ int st_field_create(st_ctx *ctx, const st_mesh *mesh, int32_t precision, st_fiel
     st_field *f = new (std::nothrow) st_field;
     if (!f) return fail(ST_E_NOMEM, "host allocation failed");
     f->ctx = ctx;
+    retain_ctx(ctx);
     f->mesh = *mesh;
     f->geo = make_geo(*mesh);
     f->prec = precision;
@@ -536,14 +573,13 @@ int st_field_create(st_ctx *ctx, const st_mesh *mesh, int32_t precision, st_fiel
 
 int st_field_destroy(st_field *f) {
     if (!f) return ST_OK;
-    if (f->ctx) {
-        hipSetDevice(f->ctx->device);
-        hipStreamSynchronize(f->ctx->stream);
-    }
+    quiesce_ctx(f->ctx);
     if (f->U) hipFree(f->U);
     if (f->src) hipFree(f->src);
     if (f->stray) hipFree(f->stray);
+    st_ctx *ctx = f->ctx;
     delete f;
+    release_ctx(ctx);
     return ST_OK;
 }
 
@@ -624,10 +660,8 @@ int st_evolve(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int32_t ns
     else
         ST_TRY(run_passes<float>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, false));
     if (bbox_out) {
-        unsigned long long keys[6];
-        ST_HIP(hipMemcpyAsync(keys, c->bbox, sizeof keys, hipMemcpyDeviceToHost, ctx->stream));
+        ST_HIP(hipMemcpyAsync(bbox_out, c->bbox, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
-        for (int k = 0; k < 6; ++k) bbox_out[k] = st::order_key_decode(keys[k]);
     }
     return ST_OK;
 }
@@ -660,6 +694,8 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
     if (c->n == 0) return fail(ST_E_INVALID, "cannot capture a step of an empty cloud");
     if (p.method == ST_METHOD_SORTED && !sort_matches(c, f, p))
         ST_TRY(c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p));
+    // every allocation happens before the capture: none is permitted inside it
+    ST_TRY(ensure_part(c, p.method == ST_METHOD_SORTED ? c->nbricks : (int32_t)((c->n + p.block - 1) / p.block)));
     ST_HIP(hipStreamSynchronize(ctx->stream));
     st_graph *g = new (std::nothrow) st_graph;
     if (!g) return fail(ST_E_NOMEM, "host allocation failed");
@@ -695,6 +731,7 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
         return fail(ST_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
     g->graph = graph;
+    retain_ctx(ctx);
     *out = g;
     return ST_OK;
 }
@@ -712,13 +749,12 @@ int st_graph_launch(st_graph *g) {
 
 int st_graph_destroy(st_graph *g) {
     if (!g) return ST_OK;
-    if (g->ctx) {
-        hipSetDevice(g->ctx->device);
-        hipStreamSynchronize(g->ctx->stream);
-    }
+    quiesce_ctx(g->ctx);
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
+    st_ctx *ctx = g->ctx;
     delete g;
+    release_ctx(ctx);
     return ST_OK;
 }
 

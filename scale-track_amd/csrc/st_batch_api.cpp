@@ -26,6 +26,7 @@ struct st_batch {
     void *descs = nullptr;           // device PassArgs<T>[n]
     std::vector<unsigned char> host_descs;
     int32_t *item0 = nullptr;        // device [n + 1]
+    st::StatsTarget *targets = nullptr;  // device [n]: statistics reduction of each cloud
     unsigned int *counter = nullptr; // device
     int32_t total_items = 0;
     int grid = 0;
@@ -58,6 +59,7 @@ int build_table(st_batch *b, double dt_sub) {
     b->host_descs.assign(n * sizeof(st::PassArgs<T>), 0);
     st::PassArgs<T> *hd = (st::PassArgs<T> *)b->host_descs.data();
     std::vector<int32_t> item0(n + 1, 0);
+    std::vector<st::StatsTarget> targets(n);
     int32_t maxcells = 1;
     for (size_t i = 0; i < n; ++i) {
         st_cloud *c = b->clouds[i];
@@ -71,9 +73,10 @@ int build_table(st_batch *b, double dt_sub) {
         a.brick_start = c->brick_start;
         a.g = f->geo;
         a.ph = make_phys<T>(c, f, dt_sub, p.cd);
-        a.st.bbox = c->bbox;
-        a.st.strays = c->strays;
+        ST_TRY(ensure_part(c, c->nbricks));
+        a.part = c->part;
         a.nbricks = c->nbricks;
+        targets[i] = st::StatsTarget{c->part, c->nbricks, c->bbox, c->strays};
         for (int k = 0; k < 3; ++k) {
             a.nb[k] = c->nb[k];
             a.bd[k] = p.brick[k];
@@ -91,6 +94,7 @@ int build_table(st_batch *b, double dt_sub) {
     hipStream_t s = b->ctx->stream;
     ST_HIP(hipMemcpyAsync(b->descs, hd, b->host_descs.size(), hipMemcpyHostToDevice, s));
     ST_HIP(hipMemcpyAsync(b->item0, item0.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    ST_HIP(hipMemcpyAsync(b->targets, targets.data(), n * sizeof(st::StatsTarget), hipMemcpyHostToDevice, s));
     ST_HIP(hipStreamSynchronize(s));
     int dev = b->ctx->device;
     int cus = 0;
@@ -117,7 +121,6 @@ int sort_all(st_batch *b) {
 int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
     ST_TRY(table_for(b, dt_sub));
     hipStream_t s = b->ctx->stream;
-    for (st_cloud *c : b->clouds) ST_HIP(st::launch_reset_stats(st::Stats{c->bbox, c->strays}, s));
     const int per = b->p.substeps_per_pass > 0 ? b->p.substeps_per_pass : nsub;
     if (!capturing) ST_HIP(hipEventRecord(b->ctx->ev[0], s));
     int launches = 0;
@@ -126,11 +129,11 @@ int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
         ST_HIP(st::launch_fill_u32(b->counter, 0u, 1, s));
         if (b->prec == ST_FP64)
             ST_HIP(st::launch_pass_batch<double>((const st::PassArgs<double> *)b->descs, b->item0,
-                                                 (int32_t)b->clouds.size(), ns, b->counter,
+                                                 (int32_t)b->clouds.size(), ns, done + ns >= nsub, b->counter,
                                                  b->p.boundary == ST_PERIODIC, b->p.block, b->grid, b->lds, s));
         else
             ST_HIP(st::launch_pass_batch<float>((const st::PassArgs<float> *)b->descs, b->item0,
-                                                (int32_t)b->clouds.size(), ns, b->counter,
+                                                (int32_t)b->clouds.size(), ns, done + ns >= nsub, b->counter,
                                                 b->p.boundary == ST_PERIODIC, b->p.block, b->grid, b->lds, s));
         ++launches;
     }
@@ -138,6 +141,7 @@ int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
         ST_HIP(hipEventRecord(b->ctx->ev[1], s));
         b->ctx->timed_launches = launches;
     }
+    ST_HIP(st::launch_reduce_stats_many(b->targets, (int32_t)b->clouds.size(), s));
     if (b->p.flush == ST_FLUSH_OWNER)
         for (st_field *f : b->fields) f->stray_dirty = true;
     return ST_OK;
@@ -145,14 +149,14 @@ int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
 
 int destroy(st_batch *b) {
     if (!b) return ST_OK;
-    if (b->ctx) {
-        (void)hipSetDevice(b->ctx->device);
-        (void)hipStreamSynchronize(b->ctx->stream);
-    }
+    quiesce_ctx(b->ctx);
     if (b->descs) (void)hipFree(b->descs);
     if (b->item0) (void)hipFree(b->item0);
+    if (b->targets) (void)hipFree(b->targets);
     if (b->counter) (void)hipFree(b->counter);
+    st_ctx *ctx = b->ctx;
     delete b;
+    release_ctx(ctx);
     return ST_OK;
 }
 
@@ -192,6 +196,7 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
     st_batch *b = new (std::nothrow) st_batch;
     if (!b) return fail(ST_E_NOMEM, "host allocation failed");
     b->ctx = ctx;
+    retain_ctx(ctx);
     b->prec = clouds[0]->prec;
     b->p = p;
     b->target = target_parcels_per_brick;
@@ -210,6 +215,7 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
     }
     hipError_t e = hipMalloc(&b->descs, (size_t)n * std::max(sizeof(st::PassArgs<double>), sizeof(st::PassArgs<float>)));
     if (e == hipSuccess) e = hipMalloc((void **)&b->item0, (size_t)(n + 1) * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->targets, (size_t)n * sizeof(st::StatsTarget));
     if (e == hipSuccess) e = hipMalloc((void **)&b->counter, sizeof(unsigned int));
     if (e != hipSuccess) {
         destroy(b);
@@ -287,6 +293,7 @@ int st_batch_graph_capture(st_batch *b, double dt_fluid, int32_t nsub, int32_t z
         return fail(ST_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
     g->graph = graph;
+    retain_ctx(b->ctx);
     *out = g;
     return ST_OK;
 }

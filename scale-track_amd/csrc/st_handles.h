@@ -38,6 +38,10 @@ struct st_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int timed_launches = 0;  // launches between ev[0] and ev[1] of the last st_evolve
+    // Clouds, fields, graphs, batches and adjoints hold a reference: st_ctx_destroy only
+    // marks the context closed while any of them is alive; the last release frees it.
+    int refs = 0;
+    bool closed = false;
 };
 
 struct st_cloud {
@@ -65,7 +69,9 @@ struct st_cloud {
     int32_t nbricks = 0;
     double sort_o[3] = {0, 0, 0}, sort_e[3] = {0, 0, 0};
     // statistics of the last launch sequence
-    unsigned long long *bbox = nullptr;  // device, 6 keys
+    double *bbox = nullptr;              // device, 6: min xyz, max xyz of the last call
+    double *part = nullptr;              // device, per-workgroup statistics slots
+    int32_t part_cap = 0;
     unsigned int *strays = nullptr;      // device
     unsigned int *bad = nullptr;         // device
     unsigned int *strays_host = nullptr; // pinned
@@ -104,6 +110,11 @@ namespace host {
 
 const char *last_error_text();
 int check_ctx(st_ctx *ctx);
+void retain_ctx(st_ctx *ctx);
+// Drops a reference; frees a closed context with no references left.
+void release_ctx(st_ctx *ctx);
+// Makes the context's device current and waits for its stream (destroy paths).
+void quiesce_ctx(st_ctx *ctx);
 int alloc_set(st_cloud *c, int which);
 int validate_params(const st_params &p);
 bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p);
@@ -121,6 +132,8 @@ inline double substep_dt(int prec, double dt_fluid, int nsub) {
 template <class T>
 inline Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd);
 int merge_source(st_field *f);
+// Statistics slots for nslots workgroups (grown on demand).
+int ensure_part(st_cloud *c, int32_t nslots);
 
 template <class T>
 inline Soa<T> soa(st_cloud *c, int which) {

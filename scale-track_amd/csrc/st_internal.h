@@ -43,10 +43,17 @@ struct Soa {
     const T *rho;  // nullptr unless per-parcel density
 };
 
-// Bounding box and stray counter of one launch sequence.
-struct Stats {
-    unsigned long long *bbox;  // 6 order-preserving keys: min xyz, max xyz
-    unsigned int *strays;      // parcels found outside their brick (sorted method)
+// Per-workgroup statistics of a pass, reduced by a separate small kernel (no
+// same-address atomics): slot k holds min xyz, max xyz of the new positions and the
+// number of parcels found outside their brick.
+constexpr int kStatSlot = 8;  // doubles per slot
+
+// Where one cloud's per-slot statistics are reduced to.
+struct StatsTarget {
+    const double *part;  // nslots * kStatSlot
+    int32_t nslots;
+    double *bbox;        // 6: min xyz, max xyz
+    unsigned int *strays;
 };
 
 template <class T>
@@ -58,7 +65,7 @@ struct PassArgs {
     const int32_t *brick_start;
     Geo g;
     Phys<T> ph;
-    Stats st;
+    double *part;    // per-workgroup statistics slots, or nullptr (no statistics this pass)
     int32_t nbricks;
     int32_t nb[3];   // bricks per direction
     int32_t bd[3];   // cells per brick per direction
@@ -90,11 +97,12 @@ template <class T>
 hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
 template <class T>
 hipError_t launch_pass_batch(const PassArgs<T> *descs, const int32_t *item0, int32_t ncloud, int32_t nsteps,
-                             unsigned int *counter, int periodic, int block, int grid, size_t lds,
+                             int stats, unsigned int *counter, int periodic, int block, int grid, size_t lds,
                              hipStream_t s);
 template <class T>
 int batch_max_blocks_per_cu(int periodic, int block, size_t lds);
-hipError_t launch_reset_stats(Stats st, hipStream_t s);
+hipError_t launch_reduce_stats(const StatsTarget &t, hipStream_t s);
+hipError_t launch_reduce_stats_many(const StatsTarget *targets, int32_t n, hipStream_t s);
 template <class T>
 hipError_t launch_brick_keys(const Soa<T> &s, uint32_t np, Geo g, const int32_t nb[3],
                              const int32_t bd[3], int periodic, uint32_t *keys,

@@ -179,30 +179,28 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     box.add(px, py, pz);  // :1362
 }
 
-// Workgroup reduction of the box and the stray count, then one atomic per value.
-// Every thread of the workgroup must call it (it contains barriers).
+// Workgroup reduction of the box and the stray count into this workgroup's statistics
+// slot (plain stores).  Every thread of the workgroup must call it (it has barriers).
 template <class T>
-__device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, const Stats &st) {
-    __shared__ double red[6][1024 / kWave];
-    __shared__ uint32_t redc[1024 / kWave];
-    double v[6];
+__device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, double *slot) {
+    __shared__ double red[7][1024 / kWave];
+    double v[7];
     for (int c = 0; c < 3; ++c) {
         v[c] = (double)box.lo[c];
         v[3 + c] = (double)box.hi[c];
     }
+    v[6] = (double)strays;
     for (int off = kWave / 2; off > 0; off >>= 1) {
         for (int c = 0; c < 3; ++c) {
             v[c] = fmin(v[c], __shfl_xor(v[c], off, kWave));
             v[3 + c] = fmax(v[3 + c], __shfl_xor(v[3 + c], off, kWave));
         }
-        strays += __shfl_xor(strays, off, kWave);
+        v[6] += __shfl_xor(v[6], off, kWave);
     }
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
     const int nw = (blockDim.x + kWave - 1) / kWave;
-    if (lane == 0) {
-        for (int c = 0; c < 6; ++c) red[c][wid] = v[c];
-        redc[wid] = strays;
-    }
+    if (lane == 0)
+        for (int c = 0; c < 7; ++c) red[c][wid] = v[c];
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < nw; ++w) {
@@ -210,15 +208,9 @@ __device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, const 
                 v[c] = fmin(v[c], red[c][w]);
                 v[3 + c] = fmax(v[3 + c], red[3 + c][w]);
             }
-            strays += redc[w];
+            v[6] += red[6][w];
         }
-        for (int c = 0; c < 3; ++c) {
-            if (v[c] <= v[3 + c]) {  // skip workgroups without parcels
-                atomicMin(st.bbox + c, order_key(v[c]));
-                atomicMax(st.bbox + 3 + c, order_key(v[3 + c]));
-            }
-        }
-        if (strays) atomicAdd(st.strays, strays);
+        for (int c = 0; c < 7; ++c) slot[c] = v[c];
     }
 }
 
@@ -233,14 +225,14 @@ __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
         load_parcel(a.s, p, a.ph.rho, in);
         step_parcel<T, PERIODIC>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
     }
-    flush_stats(box, 0u, a.st);
+    if (a.part) flush_stats(box, 0u, a.part + (size_t)blockIdx.x * kStatSlot);
 }
 
 // One brick of a sorted cloud: stage its velocities in LDS, step its parcels, add its
 // source sums to HBM once per cell.  Every thread of the workgroup must call it.
 template <class T, bool PERIODIC>
 __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t brick, int32_t nsteps,
-                                              double *lds) {
+                                              double *lds, double *part) {
     const int32_t bi = brick % a.nb[0];
     const int32_t bj = (brick / a.nb[0]) % a.nb[1];
     const int32_t bk = brick / (a.nb[0] * a.nb[1]);
@@ -321,13 +313,13 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             atomic_add_global(q + 2, a2);
         }
     }
-    flush_stats(box, env.strays, a.st);
+    if (part) flush_stats(box, env.strays, part + (size_t)brick * kStatSlot);
 }
 
 template <class T, bool PERIODIC>
 __global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
     extern __shared__ double lds[];
-    process_brick<T, PERIODIC>(a, xcd_remap(blockIdx.x, a.nbricks), a.nsteps, lds);
+    process_brick<T, PERIODIC>(a, xcd_remap(blockIdx.x, a.nbricks), a.nsteps, lds, a.part);
 }
 
 // Many clouds in one persistent launch: each workgroup takes (cloud, brick) work items
@@ -337,7 +329,7 @@ __global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
 template <class T, bool PERIODIC>
 __global__ void __launch_bounds__(1024) k_pass_batch(const PassArgs<T> *__restrict__ descs,
                                                      const int32_t *__restrict__ item0, int32_t ncloud,
-                                                     int32_t nsteps, unsigned int *counter) {
+                                                     int32_t nsteps, int stats, unsigned int *counter) {
     extern __shared__ double lds[];
     __shared__ int32_t s_item;
     const int32_t total = item0[ncloud];
@@ -356,24 +348,24 @@ __global__ void __launch_bounds__(1024) k_pass_batch(const PassArgs<T> *__restri
         }
         const int32_t c = __builtin_amdgcn_readfirstlane(lo);
         const PassArgs<T> a = descs[c];
-        process_brick<T, PERIODIC>(a, item - item0[c], nsteps, lds);
+        process_brick<T, PERIODIC>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
         __syncthreads();  // s_item is rewritten only after every thread has read it
     }
 }
 
 template <class T>
 hipError_t launch_pass_batch(const PassArgs<T> *descs, const int32_t *item0, int32_t ncloud, int32_t nsteps,
-                             unsigned int *counter, int periodic, int block, int grid, size_t lds,
+                             int stats, unsigned int *counter, int periodic, int block, int grid, size_t lds,
                              hipStream_t s) {
     if (periodic)
-        k_pass_batch<T, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, counter);
+        k_pass_batch<T, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
     else
-        k_pass_batch<T, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, counter);
+        k_pass_batch<T, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
     return hipGetLastError();
 }
-template hipError_t launch_pass_batch<float>(const PassArgs<float> *, const int32_t *, int32_t, int32_t,
+template hipError_t launch_pass_batch<float>(const PassArgs<float> *, const int32_t *, int32_t, int32_t, int,
                                              unsigned int *, int, int, int, size_t, hipStream_t);
-template hipError_t launch_pass_batch<double>(const PassArgs<double> *, const int32_t *, int32_t, int32_t,
+template hipError_t launch_pass_batch<double>(const PassArgs<double> *, const int32_t *, int32_t, int32_t, int,
                                               unsigned int *, int, int, int, size_t, hipStream_t);
 
 template <class T>
@@ -415,15 +407,53 @@ template hipError_t launch_pass_sorted<double>(const PassArgs<double> &, int, in
 // ---------------------------------------------------------------------------------
 // Small utility kernels (graph-capturable: no hipMemsetAsync anywhere).
 
-__global__ void k_reset_stats(Stats st) {
-    const int t = threadIdx.x;
-    if (t < 3) st.bbox[t] = order_key(INFINITY);
-    else if (t < 6) st.bbox[t] = order_key(-INFINITY);
-    else if (t == 6) st.strays[0] = 0u;
+// Reduces one cloud's statistics slots: one workgroup of 256 threads per target.
+__device__ __forceinline__ void reduce_stats(const StatsTarget &t) {
+    __shared__ double red[7][256 / kWave];
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int32_t k = threadIdx.x; k < t.nslots; k += blockDim.x) {
+        const double *q = t.part + (size_t)k * kStatSlot;
+        for (int c = 0; c < 3; ++c) {
+            v[c] = fmin(v[c], q[c]);
+            v[3 + c] = fmax(v[3 + c], q[3 + c]);
+        }
+        v[6] += q[6];
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        for (int c = 0; c < 3; ++c) {
+            v[c] = fmin(v[c], __shfl_xor(v[c], off, kWave));
+            v[3 + c] = fmax(v[3 + c], __shfl_xor(v[3 + c], off, kWave));
+        }
+        v[6] += __shfl_xor(v[6], off, kWave);
+    }
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    if (lane == 0)
+        for (int c = 0; c < 7; ++c) red[c][wid] = v[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) {
+            for (int c = 0; c < 3; ++c) {
+                v[c] = fmin(v[c], red[c][w]);
+                v[3 + c] = fmax(v[3 + c], red[3 + c][w]);
+            }
+            v[6] += red[6][w];
+        }
+        for (int c = 0; c < 6; ++c) t.bbox[c] = v[c];
+        t.strays[0] = (unsigned int)v[6];
+    }
 }
 
-hipError_t launch_reset_stats(Stats st, hipStream_t s) {
-    k_reset_stats<<<1, 64, 0, s>>>(st);
+__global__ void __launch_bounds__(256) k_reduce_stats(StatsTarget t) { reduce_stats(t); }
+__global__ void __launch_bounds__(256) k_reduce_stats_many(const StatsTarget *ts) { reduce_stats(ts[blockIdx.x]); }
+
+hipError_t launch_reduce_stats(const StatsTarget &t, hipStream_t s) {
+    k_reduce_stats<<<1, 256, 0, s>>>(t);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_stats_many(const StatsTarget *targets, int32_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_reduce_stats_many<<<n, 256, 0, s>>>(targets);
     return hipGetLastError();
 }
 
@@ -657,17 +687,26 @@ template hipError_t launch_range_check<float>(const Soa<float> &, uint32_t, Geo,
 template hipError_t launch_range_check<double>(const Soa<double> &, uint32_t, Geo, unsigned int *,
                                                hipStream_t);
 
-// Streaming copy (bandwidth calibration): 16-byte loads and stores, grid-stride.
+// Streaming copy (bandwidth calibration): each thread moves 4 independent 16-byte
+// vectors per iteration, so four loads are in flight before the first store.
 __global__ void __launch_bounds__(256) k_copy(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        b[i] = a[i];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const uint4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+        b[i] = v0;
+        b[i + stride] = v1;
+        b[i + 2 * stride] = v2;
+        b[i + 3 * stride] = v3;
+    }
+    for (; i < n; i += stride) b[i] = a[i];
 }
 
 hipError_t launch_copy(const void *a, void *b, size_t bytes, hipStream_t s) {
     const size_t n = bytes / 16;
     if (!n) return hipSuccess;
-    size_t g = (n + 255) / 256;
-    if (g > 256 * 64) g = 256 * 64;
+    size_t g = (n + 1023) / 1024;
+    if (g > 256 * 16) g = 256 * 16;
     k_copy<<<(unsigned)g, 256, 0, s>>>((const uint4 *)a, (uint4 *)b, n);
     return hipGetLastError();
 }
