@@ -228,7 +228,7 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
         a.nb[k] = c->nb[k];
         a.bd[k] = p.brick[k];
     }
-    a.tile_cells = p.brick[0] * p.brick[1] * p.brick[2];
+    a.tile_cells = (p.brick[0] + 2) * (p.brick[1] + 2) * (p.brick[2] + 2);
     a.flush = p.flush;
     a.np = (uint32_t)c->n;
     const int per = p.substeps_per_pass > 0 ? p.substeps_per_pass : nsub;
@@ -308,6 +308,14 @@ int ensure_part(st_cloud *c, int32_t nslots) {
     return ST_OK;
 }
 
+int check_tile(const st_ctx *ctx, int prec, const int32_t brick[3]) {
+    const size_t need = prec == ST_FP64 ? tile_lds_bytes<double>(brick) : tile_lds_bytes<float>(brick);
+    if (need > ctx->max_lds)
+        return fail(ST_E_INVALID, "brick %dx%dx%d needs %zu B of LDS with its halo; this device allows %zu",
+                    brick[0], brick[1], brick[2], need, ctx->max_lds);
+    return ST_OK;
+}
+
 int merge_source(st_field *f) {
     if (f->stray_dirty) {
         ST_HIP(launch_merge(f->src, f->stray, (size_t)f->ncell * 3, f->ctx->stream));
@@ -381,6 +389,10 @@ int st_ctx_create(int32_t device, st_ctx **out) {
     if (!c) return fail(ST_E_NOMEM, "host allocation failed");
     c->device = device;
     e = hipSetDevice(device);
+    int lds = 0;
+    if (e == hipSuccess && hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+        lds > 0)
+        c->max_lds = (size_t)lds;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&c->ev[1]);
@@ -413,6 +425,9 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
     ST_TRY(check_ctx(ctx));
     if (precision != ST_FP64 && precision != ST_FP32) return fail(ST_E_INVALID, "precision must be ST_FP64 or ST_FP32");
     if (n < 0 || n > 0x7fffffffLL) return fail(ST_E_RANGE, "parcel count %lld outside [0, 2^31)", (long long)n);
+    // each parcel array is addressed with 32-bit byte offsets on the device
+    if ((uint64_t)n * (precision == ST_FP64 ? 8u : 4u) >= (1ull << 32))
+        return fail(ST_E_RANGE, "parcel arrays of %lld parcels exceed 4 GiB; split the cloud", (long long)n);
     if (!std::isfinite(mu_c) || mu_c < 0) return fail(ST_E_INVALID, "mu_c must be finite and >= 0");
     if (!per_parcel_rho && (!std::isfinite(rho_d) || !(rho_d > 0)))
         return fail(ST_E_INVALID, "rho_d must be finite and > 0");
@@ -654,7 +669,10 @@ int st_evolve(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int32_t ns
             for (int k = 0; k < 6; ++k) bbox_out[k] = k < 3 ? INFINITY : -INFINITY;
         return ST_OK;
     }
-    if (p.method == ST_METHOD_SORTED) ST_TRY(maybe_sort(c, f, p));
+    if (p.method == ST_METHOD_SORTED) {
+        ST_TRY(check_tile(ctx, c->prec, p.brick));
+        ST_TRY(maybe_sort(c, f, p));
+    }
     if (c->prec == ST_FP64)
         ST_TRY(run_passes<double>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, false));
     else
@@ -692,6 +710,7 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
     st_params p;
     ST_TRY(prepare(ctx, c, f, dt_fluid, nsub, params, p));
     if (c->n == 0) return fail(ST_E_INVALID, "cannot capture a step of an empty cloud");
+    if (p.method == ST_METHOD_SORTED) ST_TRY(check_tile(ctx, c->prec, p.brick));
     if (p.method == ST_METHOD_SORTED && !sort_matches(c, f, p))
         ST_TRY(c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p));
     // every allocation happens before the capture: none is permitted inside it

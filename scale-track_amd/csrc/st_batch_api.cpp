@@ -36,10 +36,16 @@ struct st_batch {
 
 namespace {
 
-// Brick of about `cells` cells: powers of two, x extent largest (rows stay contiguous).
-void auto_brick(int64_t cells, const int32_t n[3], int32_t b[3]) {
+// Brick of about `cells` cells: powers of two, x extent largest (rows stay contiguous),
+// at most `max_cells` including the one-cell halo.
+void auto_brick(int64_t cells, const int32_t n[3], int64_t max_cells, int32_t b[3]) {
     int64_t c = 1;
     while (c * 2 <= cells && c < 2048) c *= 2;
+    while (c > 1) {  // largest halo tile of c cells: a cube-ish brick of side (c)^(1/3)
+        const double side = std::cbrt((double)c) + 2.0;
+        if (side * side * side * 1.2 <= (double)max_cells) break;
+        c /= 2;
+    }
     b[0] = b[1] = b[2] = 1;
     int axis = 0;
     while (c > 1) {
@@ -81,7 +87,7 @@ int build_table(st_batch *b, double dt_sub) {
             a.nb[k] = c->nb[k];
             a.bd[k] = p.brick[k];
         }
-        a.tile_cells = p.brick[0] * p.brick[1] * p.brick[2];
+        a.tile_cells = (p.brick[0] + 2) * (p.brick[1] + 2) * (p.brick[2] + 2);
         a.nsteps = 1;
         a.flush = p.flush;
         a.np = (uint32_t)c->n;
@@ -209,7 +215,18 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
         st_params pc = p;
         if (target_parcels_per_brick > 0) {
             const double want = target_parcels_per_brick / std::max(density[i], 1e-9);
-            auto_brick((int64_t)std::max(1.0, want), fields[i]->mesh.n, pc.brick);
+            const size_t per_cell = 3 * (sizeof(double) + (clouds[i]->prec == ST_FP64 ? 8 : 4));
+            auto_brick((int64_t)std::max(1.0, want), fields[i]->mesh.n, (int64_t)(ctx->max_lds / per_cell), pc.brick);
+            while (check_tile(ctx, clouds[i]->prec, pc.brick) != ST_OK && pc.brick[0] * pc.brick[1] * pc.brick[2] > 1) {
+                int ax = pc.brick[0] >= pc.brick[1] && pc.brick[0] >= pc.brick[2] ? 0 : (pc.brick[1] >= pc.brick[2] ? 1 : 2);
+                pc.brick[ax] = std::max(1, pc.brick[ax] / 2);
+            }
+        }
+        int r = check_tile(ctx, clouds[i]->prec, pc.brick);
+        if (r != ST_OK) {
+            delete b;
+            release_ctx(ctx);
+            return r;
         }
         b->cloud_params.push_back(pc);
     }

@@ -88,16 +88,29 @@ struct ParcelIn {
     T d, x, y, z, u, v, w, rho;
 };
 
+// Element access through a 32-bit byte offset from a workgroup-uniform base: the loads
+// and stores then use the scalar-base addressing form and need no 64-bit address
+// arithmetic per array.  Every parcel array is below 4 GiB (checked at cloud creation).
+template <class T>
+__device__ __forceinline__ T ld_off(const T *base, uint32_t off) {
+    return *(const T *)((const char *)base + off);
+}
+template <class T>
+__device__ __forceinline__ void st_off(T *base, uint32_t off, T v) {
+    *(T *)((char *)base + off) = v;
+}
+
 template <class T>
 __device__ __forceinline__ void load_parcel(const Soa<T> &s, uint32_t p, T rho_chunk, ParcelIn<T> &in) {
-    in.d = s.d[p];
-    in.x = s.x[p];
-    in.y = s.y[p];
-    in.z = s.z[p];
-    in.u = s.u[p];
-    in.v = s.v[p];
-    in.w = s.w[p];
-    in.rho = s.rho ? s.rho[p] : rho_chunk;
+    const uint32_t off = p * (uint32_t)sizeof(T);
+    in.d = ld_off(s.d, off);
+    in.x = ld_off(s.x, off);
+    in.y = ld_off(s.y, off);
+    in.z = ld_off(s.z, off);
+    in.u = ld_off(s.u, off);
+    in.v = ld_off(s.v, off);
+    in.w = ld_off(s.w, off);
+    in.rho = s.rho ? ld_off(s.rho, off) : rho_chunk;
 }
 
 // Kinematics of one substep (asyncParallelTracking.jl:1294-1335): implicit-Euler

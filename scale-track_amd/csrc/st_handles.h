@@ -38,6 +38,7 @@ struct st_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int timed_launches = 0;  // launches between ev[0] and ev[1] of the last st_evolve
+    size_t max_lds = 65536;  // LDS bytes one workgroup may use on this device
     // Clouds, fields, graphs, batches and adjoints hold a reference: st_ctx_destroy only
     // marks the context closed while any of them is alive; the last release frees it.
     int refs = 0;
@@ -132,6 +133,8 @@ inline double substep_dt(int prec, double dt_fluid, int nsub) {
 template <class T>
 inline Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd);
 int merge_source(st_field *f);
+// ST_E_INVALID unless the brick's halo tile fits in one workgroup's LDS.
+int check_tile(const st_ctx *ctx, int prec, const int32_t brick[3]);
 // Statistics slots for nslots workgroups (grown on demand).
 int ensure_part(st_cloud *c, int32_t nslots);
 

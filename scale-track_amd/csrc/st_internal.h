@@ -69,7 +69,7 @@ struct PassArgs {
     int32_t nbricks;
     int32_t nb[3];   // bricks per direction
     int32_t bd[3];   // cells per brick per direction
-    int32_t tile_cells;  // bd[0]*bd[1]*bd[2]
+    int32_t tile_cells;  // (bd[0]+2)*(bd[1]+2)*(bd[2]+2): a brick with a halo of one cell
     int32_t nsteps;
     int32_t flush;   // 0 owner, 1 atomic
     uint32_t np;
@@ -85,6 +85,12 @@ __host__ __device__ inline double order_key_decode(unsigned long long k) {
     union { double d; unsigned long long u; } c;
     c.u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
     return c.d;
+}
+
+// LDS bytes of the halo tile of a brick (velocities in T, sums in double).
+template <class T>
+inline size_t tile_lds_bytes(const int32_t bd[3]) {
+    return (size_t)(bd[0] + 2) * (bd[1] + 2) * (bd[2] + 2) * 3 * (sizeof(double) + sizeof(T));
 }
 
 }  // namespace st

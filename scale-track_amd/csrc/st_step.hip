@@ -42,31 +42,36 @@ struct GlobalEnv {
 // load of the tile would share the vector-memory counter with the HBM loads in flight).
 #define ST_LDS __attribute__((address_space(3)))
 
-// A brick staged in LDS; cells outside it fall back to HBM.
+// A brick staged in LDS with a halo of one cell on every side, so parcels that drifted
+// into a neighbouring brick still read their cell velocity and sum their deposits in
+// LDS.  Only parcels further away use HBM; that path waits for its own gather at once,
+// because vector-memory operations complete in order and a wait placed later would
+// also wait for the next parcel's loads and this parcel's stores.
 template <class T>
 struct TileEnv {
     const T *__restrict__ U;
     double *__restrict__ out;  // stray array (owner flush) or the source (atomic flush)
     const ST_LDS T *Ut;        // LDS velocity tile, AoS
     ST_LDS double *St;         // LDS source tile, AoS
-    int32_t o0, o1, o2;        // tile origin (cells)
-    uint32_t e0, e1, e2;       // tile extent (cells)
+    int32_t h0, h1, h2;        // tile origin (cells) = brick origin - 1
+    uint32_t w0, w1, w2;       // tile extent = brick extent + 2
+    uint32_t e0, e1, e2;       // brick extent
     uint32_t strays;
     __device__ __forceinline__ bool local(const Cell &cl, uint32_t &l) const {
-        const uint32_t a = (uint32_t)(cl.f[0] - o0);
-        const uint32_t b = (uint32_t)(cl.f[1] - o1);
-        const uint32_t c = (uint32_t)(cl.f[2] - o2);
-        l = a + e0 * (b + e1 * c);
-        return a < e0 && b < e1 && c < e2;
+        const uint32_t a = (uint32_t)(cl.f[0] - h0);
+        const uint32_t b = (uint32_t)(cl.f[1] - h1);
+        const uint32_t c = (uint32_t)(cl.f[2] - h2);
+        l = a + w0 * (b + w1 * c);
+        return a < w0 && b < w1 && c < w2;
     }
     __device__ __forceinline__ void enter(const Cell &cl) {
-        uint32_t l;
-        if (!local(cl, l)) ++strays;
+        const uint32_t a = (uint32_t)(cl.f[0] - h0 - 1);
+        const uint32_t b = (uint32_t)(cl.f[1] - h1 - 1);
+        const uint32_t c = (uint32_t)(cl.f[2] - h2 - 1);
+        if (!(a < e0 && b < e1 && c < e2)) ++strays;
     }
     __device__ __forceinline__ void load_u(const Cell &cl, T &ux, T &uy, T &uz) const {
         uint32_t l;
-        // Two explicit paths: a select between an LDS and a global pointer would
-        // make the compiler spill both into a private array.
         if (local(cl, l)) {
             ux = Ut[3u * l];
             uy = Ut[3u * l + 1];
@@ -76,6 +81,7 @@ struct TileEnv {
             ux = q[0];
             uy = q[1];
             uz = q[2];
+            __builtin_amdgcn_s_waitcnt(0);
         }
     }
     __device__ __forceinline__ void deposit(const Cell &cl, double a, double b, double c) {
@@ -140,8 +146,6 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     T ux, uy, uz;
     env.load_u(cell, ux, uy, uz);  // :1287-1288
     double s0x = (double)vx, s0y = (double)vy, s0z = (double)vz;  // :1289
-    // Consume the velocity before the prefetch so its wait cannot cover the new loads.
-    __builtin_amdgcn_s_waitcnt(0);
     prefetch();
 
     for (int32_t t = 1; t <= nsteps; ++t) {
@@ -159,7 +163,6 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
             s0z = (double)nvz;
             cell = nc;
             env.load_u(cell, ux, uy, uz);
-            __builtin_amdgcn_s_waitcnt(0);  // wait here, not at every substep
         }
         vx = nvx;  // :1351-1352
         vy = nvy;
@@ -169,14 +172,90 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
         pz = nz;
     }
     // :1355-1361
-    s.x[p] = px;
-    s.y[p] = py;
-    s.z[p] = pz;
-    s.u[p] = vx;
-    s.v[p] = vy;
-    s.w[p] = vz;
+    const uint32_t off = p * (uint32_t)sizeof(T);
+    st_off(s.x, off, px);
+    st_off(s.y, off, py);
+    st_off(s.z, off, pz);
+    st_off(s.u, off, vx);
+    st_off(s.v, off, vy);
+    st_off(s.w, off, vz);
     env.deposit(cell, m * (s0x - (double)vx), m * (s0y - (double)vy), m * (s0z - (double)vz));
     box.add(px, py, pz);  // :1362
+}
+
+// The same step for a pass of exactly one substep (the schedule of one launch per
+// substep).  With t == nSteps the reference never deposits or refreshes the velocity
+// inside the loop (:1342) and never uses the new cell, so the relocation after a bounce
+// or wrap (:1333-1335) has no effect on any output and is left out; every value that
+// is written is computed by the same operations as in step_parcel.
+template <class T, bool PERIODIC, class Env, class Prefetch>
+__device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
+                                                const ParcelIn<T> &in, uint32_t p, Env &env, Box<T> &box,
+                                                bool stats, const Prefetch &prefetch) {
+    double m;
+    T f, den;
+    parcel_consts(ph, s.rho != nullptr, in.d, in.rho, m, f, den);
+    const T px = in.x, py = in.y, pz = in.z;
+    const T vx = in.u, vy = in.v, vz = in.w;
+
+    int32_t I[3];
+    Cell cell;
+    locate<T, PERIODIC>(g, px, py, pz, I, cell);  // :1286
+    env.enter(cell);
+    T ux, uy, uz;
+    env.load_u(cell, ux, uy, uz);  // :1287-1288
+    prefetch();
+
+    // :1295-1296
+    T nvx = (vx + f * ux) / den;
+    T nvy = (vy + f * uy) / den;
+    T nvz = (vz + f * uz) / den;
+    T nx = px + nvx * ph.dt;
+    T ny = py + nvy * ph.dt;
+    T nz = pz + nvz * ph.dt;
+    double s0x = (double)vx, s0y = (double)vy, s0z = (double)vz;  // :1289
+    if (!PERIODIC) {
+        Cell nc;
+        locate<T, false>(g, nx, ny, nz, I, nc);  // :1303
+        if (nc.c != cell.c || nc.part != cell.part) {  // :1306
+            if (I[0] < 0 || I[0] >= g.n[0]) {  // :1309-1329, :730-736
+                nx = px;
+                nvx = -vx;
+                s0x = s0x - 2.0 * (double)vx;
+            }
+            if (I[1] < 0 || I[1] >= g.n[1]) {
+                ny = py;
+                nvy = -vy;
+                s0y = s0y - 2.0 * (double)vy;
+            }
+            if (I[2] < 0 || I[2] >= g.n[2]) {
+                nz = pz;
+                nvz = -vz;
+                s0z = s0z - 2.0 * (double)vz;
+            }
+        }
+    } else {
+        // periodic extension: shift each axis that left [0, N) by one period
+        const int32_t i0 = axis_floor((double)nx, g.o[0], g.rd[0]);
+        const int32_t i1 = axis_floor((double)ny, g.o[1], g.rd[1]);
+        const int32_t i2 = axis_floor((double)nz, g.o[2], g.rd[2]);
+        if (i0 < 0) nx = nx + ph.L[0];
+        else if (i0 >= g.n[0]) nx = nx - ph.L[0];
+        if (i1 < 0) ny = ny + ph.L[1];
+        else if (i1 >= g.n[1]) ny = ny - ph.L[1];
+        if (i2 < 0) nz = nz + ph.L[2];
+        else if (i2 >= g.n[2]) nz = nz - ph.L[2];
+    }
+    // :1355-1361
+    const uint32_t off = p * (uint32_t)sizeof(T);
+    st_off(s.x, off, nx);
+    st_off(s.y, off, ny);
+    st_off(s.z, off, nz);
+    st_off(s.u, off, nvx);
+    st_off(s.v, off, nvy);
+    st_off(s.w, off, nvz);
+    env.deposit(cell, m * (s0x - (double)nvx), m * (s0y - (double)nvy), m * (s0z - (double)nvz));
+    if (stats) box.add(nx, ny, nz);  // :1362
 }
 
 // Workgroup reduction of the box and the stray count into this workgroup's statistics
@@ -223,7 +302,10 @@ __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
     if (p < a.np) {
         ParcelIn<T> in;
         load_parcel(a.s, p, a.ph.rho, in);
-        step_parcel<T, PERIODIC>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
+        if (a.nsteps == 1)
+            step_parcel_one<T, PERIODIC>(a.g, a.ph, a.s, in, p, env, box, true, NoPrefetch{});
+        else
+            step_parcel<T, PERIODIC>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
     }
     if (a.part) flush_stats(box, 0u, a.part + (size_t)blockIdx.x * kStatSlot);
 }
@@ -240,34 +322,46 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     TileEnv<T> env;
     env.U = a.U;
     env.out = a.flush == 0 ? a.stray : a.src;
-    env.o0 = bi * a.bd[0];
-    env.o1 = bj * a.bd[1];
-    env.o2 = bk * a.bd[2];
-    const int32_t ex = min(a.bd[0], a.g.n[0] - env.o0);
-    const int32_t ey = min(a.bd[1], a.g.n[1] - env.o1);
-    const int32_t ez = min(a.bd[2], a.g.n[2] - env.o2);
+    const int32_t o0 = bi * a.bd[0], o1 = bj * a.bd[1], o2 = bk * a.bd[2];
+    const int32_t ex = min(a.bd[0], a.g.n[0] - o0);
+    const int32_t ey = min(a.bd[1], a.g.n[1] - o1);
+    const int32_t ez = min(a.bd[2], a.g.n[2] - o2);
+    env.h0 = o0 - 1;
+    env.h1 = o1 - 1;
+    env.h2 = o2 - 1;
+    const int32_t wx = ex + 2, wy = ey + 2, wz = ez + 2;
+    env.w0 = (uint32_t)wx;
+    env.w1 = (uint32_t)wy;
+    env.w2 = (uint32_t)wz;
     env.e0 = (uint32_t)ex;
     env.e1 = (uint32_t)ey;
     env.e2 = (uint32_t)ez;
     env.strays = 0;
-    const int32_t ncell = ex * ey * ez;
+    const int32_t ncell = wx * wy * wz;
     ST_LDS double *St = (ST_LDS double *)lds;
     ST_LDS T *Ut = (ST_LDS T *)(St + 3 * a.tile_cells);
     env.St = St;
     env.Ut = Ut;
 
-    // Stage the brick's velocities (rows of e[0] contiguous cells) and clear its sums.
+    // Stage the tile's velocities (halo cells outside the mesh are never looked up:
+    // every cell index is folded or clamped into the mesh) and clear its sums.
     for (int32_t t = threadIdx.x; t < ncell; t += blockDim.x) {
-        const int32_t li = t % ex;
-        const int32_t r = t / ex;
-        const int32_t lj = r % ey;
-        const int32_t lk = r / ey;
-        const uint32_t c = (uint32_t)(env.o2 + lk) * a.g.nxny +
-                           (uint32_t)(env.o1 + lj) * (uint32_t)a.g.n[0] + (uint32_t)(env.o0 + li);
-        const T *q = a.U + 3u * c;
-        Ut[3 * t] = q[0];
-        Ut[3 * t + 1] = q[1];
-        Ut[3 * t + 2] = q[2];
+        const int32_t li = t % wx;
+        const int32_t r = t / wx;
+        const int32_t lj = r % wy;
+        const int32_t lk = r / wy;
+        const int32_t gi = env.h0 + li, gj = env.h1 + lj, gk = env.h2 + lk;
+        T u0 = (T)0, u1 = (T)0, u2 = (T)0;
+        if ((uint32_t)gi < (uint32_t)a.g.n[0] && (uint32_t)gj < (uint32_t)a.g.n[1] &&
+            (uint32_t)gk < (uint32_t)a.g.n[2]) {
+            const T *q = a.U + 3u * ((uint32_t)gk * a.g.nxny + (uint32_t)gj * (uint32_t)a.g.n[0] + (uint32_t)gi);
+            u0 = q[0];
+            u1 = q[1];
+            u2 = q[2];
+        }
+        Ut[3 * t] = u0;
+        Ut[3 * t + 1] = u1;
+        Ut[3 * t + 2] = u2;
         St[3 * t] = 0.0;
         St[3 * t + 1] = 0.0;
         St[3 * t + 2] = 0.0;
@@ -281,33 +375,48 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     int32_t p = p0 + (int32_t)threadIdx.x;
     ParcelIn<T> cur, nxt;
     if (p < p1) load_parcel(a.s, (uint32_t)p, a.ph.rho, cur);
-    for (; p < p1; p += blockDim.x) {
-        const int32_t q = p + (int32_t)blockDim.x;
-        step_parcel<T, PERIODIC>(a.g, a.ph, a.s, cur, (uint32_t)p, nsteps, env, box, [&] {
-            if (q < p1) load_parcel(a.s, (uint32_t)q, a.ph.rho, nxt);
-        });
-        cur = nxt;
+    if (nsteps == 1) {
+        const bool stats = part != nullptr;
+        for (; p < p1; p += blockDim.x) {
+            const int32_t q = p + (int32_t)blockDim.x;
+            step_parcel_one<T, PERIODIC>(a.g, a.ph, a.s, cur, (uint32_t)p, env, box, stats, [&] {
+                if (q < p1) load_parcel(a.s, (uint32_t)q, a.ph.rho, nxt);
+            });
+            cur = nxt;
+        }
+    } else {
+        for (; p < p1; p += blockDim.x) {
+            const int32_t q = p + (int32_t)blockDim.x;
+            step_parcel<T, PERIODIC>(a.g, a.ph, a.s, cur, (uint32_t)p, nsteps, env, box, [&] {
+                if (q < p1) load_parcel(a.s, (uint32_t)q, a.ph.rho, nxt);
+            });
+            cur = nxt;
+        }
     }
     __syncthreads();
 
-    // Add the brick's sums to HBM once per cell.  With the owner flush only this
-    // workgroup touches these cells of `src` during the launch (strays went to
-    // `stray`), so a plain read-modify-write is race-free.
+    // Add the tile's sums to HBM.  With the owner flush only this workgroup touches the
+    // brick's own cells of `src` during the launch (deposits from other workgroups'
+    // halos and strays go to `stray`), so a plain read-modify-write is race-free there;
+    // the halo's sums go to `stray` with atomics.
     for (int32_t t = threadIdx.x; t < ncell; t += blockDim.x) {
         const double a0 = St[3 * t], a1 = St[3 * t + 1], a2 = St[3 * t + 2];
         if (a0 == 0.0 && a1 == 0.0 && a2 == 0.0) continue;
-        const int32_t li = t % ex;
-        const int32_t r = t / ex;
-        const int32_t lj = r % ey;
-        const int32_t lk = r / ey;
-        const uint32_t c = (uint32_t)(env.o2 + lk) * a.g.nxny +
-                           (uint32_t)(env.o1 + lj) * (uint32_t)a.g.n[0] + (uint32_t)(env.o0 + li);
-        double *q = a.src + 3u * c;
-        if (a.flush == 0) {
+        const int32_t li = t % wx;
+        const int32_t r = t / wx;
+        const int32_t lj = r % wy;
+        const int32_t lk = r / wy;
+        const int32_t gi = env.h0 + li, gj = env.h1 + lj, gk = env.h2 + lk;
+        const uint32_t c = (uint32_t)gk * a.g.nxny + (uint32_t)gj * (uint32_t)a.g.n[0] + (uint32_t)gi;
+        const bool own = (uint32_t)(li - 1) < (uint32_t)ex && (uint32_t)(lj - 1) < (uint32_t)ey &&
+                         (uint32_t)(lk - 1) < (uint32_t)ez;
+        if (own && a.flush == 0) {
+            double *q = a.src + 3u * c;
             q[0] += a0;
             q[1] += a1;
             q[2] += a2;
         } else {
+            double *q = env.out + 3u * c;
             atomic_add_global(q, a0);
             atomic_add_global(q + 1, a1);
             atomic_add_global(q + 2, a2);
