@@ -9,18 +9,18 @@
 namespace st {
 
 constexpr int kWave = 64;
-constexpr int kXcd = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
-
-// Workgroup id -> brick id so that each XCD (and its L2) gets one contiguous range of
-// bricks instead of every 8th brick.
-__device__ __forceinline__ int32_t xcd_remap(int32_t bid, int32_t n) {
-    const int32_t q = n / kXcd, r = n % kXcd;
-    const int32_t x = bid % kXcd, j = bid / kXcd;
-    return x * q + (x < r ? x : r) + j;
-}
 
 __device__ __forceinline__ int32_t axis_floor(double x, double o, double rd) {
     return (int32_t)floor((x - o) * rd);
+}
+
+// Which side of the mesh an axis coordinate lies on: -1 below, +1 at or above the end,
+// 0 inside.  floor(t) < 0 exactly when t < 0, and floor(t) >= n exactly when t >= n
+// (n an integer), so this takes the decisions axis_floor's index would take, with
+// the floor and the integer conversion left out.
+__device__ __forceinline__ int axis_side(double x, double o, double rd, int32_t n) {
+    const double t = (x - o) * rd;
+    return t < 0.0 ? -1 : (t >= (double)n ? 1 : 0);
 }
 
 // divrem by the partition size (= N for one partition) with truncation toward zero,
@@ -47,13 +47,25 @@ struct Cell {
     int32_t part;  // folded partition index (wall mode)
 };
 
+// Linear cell of folded indices; kernels that need it only off their fast path call
+// this there instead of reading Cell::c (two quarter-rate integer multiplies).
+__device__ __forceinline__ uint32_t lin(const Geo &g, const Cell &cl) {
+    return (uint32_t)cl.f[2] * g.nxny + (uint32_t)cl.f[1] * (uint32_t)g.n[0] + (uint32_t)cl.f[0];
+}
+
+// Same folded cell: the linear index is one-to-one on folded indices, so comparing the
+// three axes decides what comparing Cell::c does.
+__device__ __forceinline__ bool same_cell(const Cell &a, const Cell &b) {
+    return a.f[0] == b.f[0] && a.f[1] == b.f[1] && a.f[2] == b.f[2];
+}
+
 __device__ __forceinline__ void cell_wall(const Geo &g, const int32_t I[3], Cell &cl) {
     int32_t p0, p1, p2;
     fold(I[0], g.n[0], cl.f[0], p0);
     fold(I[1], g.n[1], cl.f[1], p1);
     fold(I[2], g.n[2], cl.f[2], p2);
     cl.part = p0 + p1 + p2;
-    cl.c = (uint32_t)cl.f[2] * g.nxny + (uint32_t)cl.f[1] * (uint32_t)g.n[0] + (uint32_t)cl.f[0];
+    cl.c = lin(g, cl);
 }
 
 __device__ __forceinline__ void cell_clamped(const Geo &g, const int32_t I[3], Cell &cl) {
@@ -61,7 +73,7 @@ __device__ __forceinline__ void cell_clamped(const Geo &g, const int32_t I[3], C
     cl.f[1] = clampi(I[1], g.n[1]);
     cl.f[2] = clampi(I[2], g.n[2]);
     cl.part = 0;
-    cl.c = (uint32_t)cl.f[2] * g.nxny + (uint32_t)cl.f[1] * (uint32_t)g.n[0] + (uint32_t)cl.f[0];
+    cl.c = lin(g, cl);
 }
 
 template <class T, bool PERIODIC>
@@ -100,7 +112,11 @@ __device__ __forceinline__ void st_off(T *base, uint32_t off, T v) {
     *(T *)((char *)base + off) = v;
 }
 
-template <class T>
+// RHO: the cloud carries a per-parcel density array.  It is a template parameter, not
+// a test of s.rho, so that no load in the hot loops is conditional: the compiler's
+// wait-count placement treats a register that a load may or may not have written as
+// pending and drains every load in flight before its first use.
+template <class T, bool RHO>
 __device__ __forceinline__ void load_parcel(const Soa<T> &s, uint32_t p, T rho_chunk, ParcelIn<T> &in) {
     const uint32_t off = p * (uint32_t)sizeof(T);
     in.d = ld_off(s.d, off);
@@ -110,7 +126,7 @@ __device__ __forceinline__ void load_parcel(const Soa<T> &s, uint32_t p, T rho_c
     in.u = ld_off(s.u, off);
     in.v = ld_off(s.v, off);
     in.w = ld_off(s.w, off);
-    in.rho = s.rho ? ld_off(s.rho, off) : rho_chunk;
+    in.rho = RHO ? ld_off(s.rho, off) : rho_chunk;
 }
 
 // Kinematics of one substep (asyncParallelTracking.jl:1294-1335): implicit-Euler
@@ -134,7 +150,7 @@ __device__ __forceinline__ unsigned advance(const Geo &g, const Phys<T> &ph, T f
     unsigned bounced = 0;
     if (!PERIODIC) {
         locate<T, false>(g, nx, ny, nz, I, nc);  // :1303
-        if (nc.c != cell.c || nc.part != part0) {  // :1306
+        if (!same_cell(nc, cell) || nc.part != part0) {  // :1306
             if (I[0] < 0 || I[0] >= g.n[0]) {  // :1309-1329
                 nx = px;
                 nvx = -vx;

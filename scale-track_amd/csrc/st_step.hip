@@ -57,12 +57,18 @@ struct TileEnv {
     uint32_t w0, w1, w2;       // tile extent = brick extent + 2
     uint32_t e0, e1, e2;       // brick extent
     uint32_t strays;
+    uint32_t nxny, nx;         // mesh strides (far path only)
+    // Tile index of a cell.  The tile extents are far below 2^24, so the index is formed
+    // with full-rate 24-bit multiplies; it is garbage (and unused) outside the tile.
     __device__ __forceinline__ bool local(const Cell &cl, uint32_t &l) const {
         const uint32_t a = (uint32_t)(cl.f[0] - h0);
         const uint32_t b = (uint32_t)(cl.f[1] - h1);
         const uint32_t c = (uint32_t)(cl.f[2] - h2);
-        l = a + w0 * (b + w1 * c);
+        l = a + __umul24(w0, b + __umul24(w1, c));
         return a < w0 && b < w1 && c < w2;
+    }
+    __device__ __forceinline__ uint32_t global(const Cell &cl) const {
+        return (uint32_t)cl.f[2] * nxny + (uint32_t)cl.f[1] * nx + (uint32_t)cl.f[0];
     }
     __device__ __forceinline__ void enter(const Cell &cl) {
         const uint32_t a = (uint32_t)(cl.f[0] - h0 - 1);
@@ -77,7 +83,7 @@ struct TileEnv {
             uy = Ut[3u * l + 1];
             uz = Ut[3u * l + 2];
         } else {
-            const T *q = U + 3u * cl.c;
+            const T *q = U + 3u * global(cl);
             ux = q[0];
             uy = q[1];
             uz = q[2];
@@ -92,7 +98,7 @@ struct TileEnv {
             atomic_add_lds(q + 1, b);
             atomic_add_lds(q + 2, c);
         } else {
-            double *q = out + 3u * cl.c;
+            double *q = out + 3u * global(cl);
             atomic_add_global(q, a);
             atomic_add_global(q + 1, b);
             atomic_add_global(q + 2, c);
@@ -109,13 +115,15 @@ struct Box {
             hi[c] = -(T)INFINITY;
         }
     }
+    // Compare-and-select: a NaN coordinate is skipped, as fmin/fmax skip it, without
+    // the operand canonicalisation fmin/fmax cost.
     __device__ __forceinline__ void add(T x, T y, T z) {
-        lo[0] = fmin(lo[0], x);
-        lo[1] = fmin(lo[1], y);
-        lo[2] = fmin(lo[2], z);
-        hi[0] = fmax(hi[0], x);
-        hi[1] = fmax(hi[1], y);
-        hi[2] = fmax(hi[2], z);
+        lo[0] = x < lo[0] ? x : lo[0];
+        lo[1] = y < lo[1] ? y : lo[1];
+        lo[2] = z < lo[2] ? z : lo[2];
+        hi[0] = x > hi[0] ? x : hi[0];
+        hi[1] = y > hi[1] ? y : hi[1];
+        hi[2] = z > hi[2] ? z : hi[2];
     }
 };
 
@@ -127,13 +135,13 @@ struct NoPrefetch {
 // once the parcel's starting cell velocity has been read, so loads it issues (the next
 // parcel's state) stay in flight across the substeps: vector-memory loads complete in
 // order, and a later HBM gather would otherwise wait for them.
-template <class T, bool PERIODIC, class Env, class Prefetch>
+template <class T, bool PERIODIC, bool RHO, class Env, class Prefetch>
 __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
                                             const ParcelIn<T> &in, uint32_t p, int32_t nsteps,
                                             Env &env, Box<T> &box, const Prefetch &prefetch) {
     double m;
     T f, den;
-    parcel_consts(ph, s.rho != nullptr, in.d, in.rho, m, f, den);
+    parcel_consts(ph, RHO, in.d, in.rho, m, f, den);
 
     T px = in.x, py = in.y, pz = in.z;
     T vx = in.u, vy = in.v, vz = in.w;
@@ -156,7 +164,7 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
         if (b & 1u) s0x = s0x - 2.0 * (double)vx;  // update_at_boundary!, :730-736
         if (b & 2u) s0y = s0y - 2.0 * (double)vy;
         if (b & 4u) s0z = s0z - 2.0 * (double)vz;
-        if (t != nsteps && nc.c != cell.c) {  // :1342-1348
+        if (t != nsteps && !same_cell(nc, cell)) {  // :1342-1348
             env.deposit(cell, m * (s0x - (double)nvx), m * (s0y - (double)nvy), m * (s0z - (double)nvz));
             s0x = (double)nvx;
             s0y = (double)nvy;
@@ -188,13 +196,13 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
 // inside the loop (:1342) and never uses the new cell, so the relocation after a bounce
 // or wrap (:1333-1335) has no effect on any output and is left out; every value that
 // is written is computed by the same operations as in step_parcel.
-template <class T, bool PERIODIC, class Env, class Prefetch>
+template <class T, bool PERIODIC, bool RHO, class Env, class Prefetch>
 __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
                                                 const ParcelIn<T> &in, uint32_t p, Env &env, Box<T> &box,
                                                 bool stats, const Prefetch &prefetch) {
     double m;
     T f, den;
-    parcel_consts(ph, s.rho != nullptr, in.d, in.rho, m, f, den);
+    parcel_consts(ph, RHO, in.d, in.rho, m, f, den);
     const T px = in.x, py = in.y, pz = in.z;
     const T vx = in.u, vy = in.v, vz = in.w;
 
@@ -217,7 +225,7 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
     if (!PERIODIC) {
         Cell nc;
         locate<T, false>(g, nx, ny, nz, I, nc);  // :1303
-        if (nc.c != cell.c || nc.part != cell.part) {  // :1306
+        if (!same_cell(nc, cell) || nc.part != cell.part) {  // :1306
             if (I[0] < 0 || I[0] >= g.n[0]) {  // :1309-1329, :730-736
                 nx = px;
                 nvx = -vx;
@@ -236,15 +244,15 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
         }
     } else {
         // periodic extension: shift each axis that left [0, N) by one period
-        const int32_t i0 = axis_floor((double)nx, g.o[0], g.rd[0]);
-        const int32_t i1 = axis_floor((double)ny, g.o[1], g.rd[1]);
-        const int32_t i2 = axis_floor((double)nz, g.o[2], g.rd[2]);
-        if (i0 < 0) nx = nx + ph.L[0];
-        else if (i0 >= g.n[0]) nx = nx - ph.L[0];
-        if (i1 < 0) ny = ny + ph.L[1];
-        else if (i1 >= g.n[1]) ny = ny - ph.L[1];
-        if (i2 < 0) nz = nz + ph.L[2];
-        else if (i2 >= g.n[2]) nz = nz - ph.L[2];
+        const int s0 = axis_side((double)nx, g.o[0], g.rd[0], g.n[0]);
+        const int s1 = axis_side((double)ny, g.o[1], g.rd[1], g.n[1]);
+        const int s2 = axis_side((double)nz, g.o[2], g.rd[2], g.n[2]);
+        if (s0 < 0) nx = nx + ph.L[0];
+        else if (s0 > 0) nx = nx - ph.L[0];
+        if (s1 < 0) ny = ny + ph.L[1];
+        else if (s1 > 0) ny = ny - ph.L[1];
+        if (s2 < 0) nz = nz + ph.L[2];
+        else if (s2 > 0) nz = nz - ph.L[2];
     }
     // :1355-1361
     const uint32_t off = p * (uint32_t)sizeof(T);
@@ -293,7 +301,7 @@ __device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, double
     }
 }
 
-template <class T, bool PERIODIC>
+template <class T, bool PERIODIC, bool RHO>
 __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     Box<T> box;
@@ -301,18 +309,37 @@ __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
     GlobalEnv<T> env{a.U, a.src};
     if (p < a.np) {
         ParcelIn<T> in;
-        load_parcel(a.s, p, a.ph.rho, in);
+        load_parcel<T, RHO>(a.s, p, a.ph.rho, in);
         if (a.nsteps == 1)
-            step_parcel_one<T, PERIODIC>(a.g, a.ph, a.s, in, p, env, box, true, NoPrefetch{});
+            step_parcel_one<T, PERIODIC, RHO>(a.g, a.ph, a.s, in, p, env, box, true, NoPrefetch{});
         else
-            step_parcel<T, PERIODIC>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
+            step_parcel<T, PERIODIC, RHO>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
     }
     if (a.part) flush_stats(box, 0u, a.part + (size_t)blockIdx.x * kStatSlot);
 }
 
+// Tile cells handled per thread per round of the staging and flush loops.
+constexpr int kStage = 4;
+
+// Index -> (i, j, k) of a tile of extent wx * wy * (any) cells, x fastest.  The
+// quotients come from a float reciprocal: for t below 2^22 the rounding error of
+// (t + 1/2) * (1/w) stays far below the 1/(2w) distance of t + 1/2 from a multiple
+// of w, so truncation gives the exact integer quotient.
+struct TileShape {
+    int32_t wx, wxy;
+    float rwx, rwxy;
+    __device__ TileShape(int32_t x, int32_t y) : wx(x), wxy(x * y), rwx(1.0f / (float)x), rwxy(1.0f / (float)(x * y)) {}
+    __device__ __forceinline__ void split(int32_t t, int32_t &i, int32_t &j, int32_t &k) const {
+        k = (int32_t)(((float)t + 0.5f) * rwxy);
+        const int32_t r = t - k * wxy;
+        j = (int32_t)(((float)r + 0.5f) * rwx);
+        i = r - j * wx;
+    }
+};
+
 // One brick of a sorted cloud: stage its velocities in LDS, step its parcels, add its
 // source sums to HBM once per cell.  Every thread of the workgroup must call it.
-template <class T, bool PERIODIC>
+template <class T, bool PERIODIC, bool RHO>
 __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t brick, int32_t nsteps,
                                               double *lds, double *part) {
     const int32_t bi = brick % a.nb[0];
@@ -337,98 +364,179 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     env.e1 = (uint32_t)ey;
     env.e2 = (uint32_t)ez;
     env.strays = 0;
+    env.nxny = a.g.nxny;
+    env.nx = (uint32_t)a.g.n[0];
     const int32_t ncell = wx * wy * wz;
     ST_LDS double *St = (ST_LDS double *)lds;
     ST_LDS T *Ut = (ST_LDS T *)(St + 3 * a.tile_cells);
     env.St = St;
     env.Ut = Ut;
 
-    // Stage the tile's velocities (halo cells outside the mesh are never looked up:
-    // every cell index is folded or clamped into the mesh) and clear its sums.
-    for (int32_t t = threadIdx.x; t < ncell; t += blockDim.x) {
-        const int32_t li = t % wx;
-        const int32_t r = t / wx;
-        const int32_t lj = r % wy;
-        const int32_t lk = r / wy;
-        const int32_t gi = env.h0 + li, gj = env.h1 + lj, gk = env.h2 + lk;
-        T u0 = (T)0, u1 = (T)0, u2 = (T)0;
-        if ((uint32_t)gi < (uint32_t)a.g.n[0] && (uint32_t)gj < (uint32_t)a.g.n[1] &&
-            (uint32_t)gk < (uint32_t)a.g.n[2]) {
-            const T *q = a.U + 3u * ((uint32_t)gk * a.g.nxny + (uint32_t)gj * (uint32_t)a.g.n[0] + (uint32_t)gi);
-            u0 = q[0];
-            u1 = q[1];
-            u2 = q[2];
+    const int32_t p0 = a.brick_start[brick], p1 = a.brick_start[brick + 1];
+    const int32_t bs = (int32_t)blockDim.x;
+    const uint32_t last = (uint32_t)(p1 - 1);
+    int32_t p = p0 + (int32_t)threadIdx.x;
+    // The first parcels' state is requested before the tile is staged, so its latency
+    // overlaps the staging (the brick is non-empty: a clamped index is valid).
+    ParcelIn<T> A, B, C;
+    if (p1 > p0) {
+        load_parcel<T, RHO>(a.s, min((uint32_t)p, last), a.ph.rho, A);
+        load_parcel<T, RHO>(a.s, min((uint32_t)(p + bs), last), a.ph.rho, B);
+    }
+
+    // Stage the tile's velocities and clear its sums, kStage cells per thread at a
+    // time with every load issued before the first LDS write (one memory round trip per
+    // kStage*blockDim cells).  Halo cells outside the mesh are never looked up (every
+    // cell index is folded or clamped into the mesh): they read a clamped cell and
+    // store zeros, so the loads need no condition.  Two lanes may write the same cell
+    // (the tile's last) with the same values: no barrier separates LDS writes of equal
+    // data, and the barrier below orders them before any read.
+    const TileShape ts(wx, wy);
+    for (int32_t t0 = 0; t0 < ncell; t0 += kStage * bs) {
+        T u[kStage][3];
+        bool in[kStage];
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int32_t t = min(t0 + k * bs + (int32_t)threadIdx.x, ncell - 1);
+            int32_t li, lj, lk;
+            ts.split(t, li, lj, lk);
+            const int32_t gi = env.h0 + li, gj = env.h1 + lj, gk = env.h2 + lk;
+            in[k] = (uint32_t)gi < (uint32_t)a.g.n[0] && (uint32_t)gj < (uint32_t)a.g.n[1] &&
+                    (uint32_t)gk < (uint32_t)a.g.n[2];
+            const uint32_t c = (uint32_t)clampi(gk, a.g.n[2]) * a.g.nxny +
+                               (uint32_t)clampi(gj, a.g.n[1]) * (uint32_t)a.g.n[0] + (uint32_t)clampi(gi, a.g.n[0]);
+            const T *q = a.U + 3u * c;
+            u[k][0] = q[0];
+            u[k][1] = q[1];
+            u[k][2] = q[2];
         }
-        Ut[3 * t] = u0;
-        Ut[3 * t + 1] = u1;
-        Ut[3 * t + 2] = u2;
-        St[3 * t] = 0.0;
-        St[3 * t + 1] = 0.0;
-        St[3 * t + 2] = 0.0;
+        // Lanes past the tile rewrite its last cell with the same values, so every load
+        // is consumed on every path and none is left in flight around the loop.
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int32_t t = min(t0 + k * bs + (int32_t)threadIdx.x, ncell - 1);
+            Ut[3 * t] = in[k] ? u[k][0] : (T)0;
+            Ut[3 * t + 1] = in[k] ? u[k][1] : (T)0;
+            Ut[3 * t + 2] = in[k] ? u[k][2] : (T)0;
+            St[3 * t] = 0.0;
+            St[3 * t + 1] = 0.0;
+            St[3 * t + 2] = 0.0;
+        }
     }
     __syncthreads();
 
     Box<T> box;
     box.init();
-    const int32_t p0 = a.brick_start[brick], p1 = a.brick_start[brick + 1];
-    // Software pipeline: the next parcel's state is in flight while this one steps.
-    int32_t p = p0 + (int32_t)threadIdx.x;
-    ParcelIn<T> cur, nxt;
-    if (p < p1) load_parcel(a.s, (uint32_t)p, a.ph.rho, cur);
-    if (nsteps == 1) {
-        const bool stats = part != nullptr;
-        for (; p < p1; p += blockDim.x) {
-            const int32_t q = p + (int32_t)blockDim.x;
-            step_parcel_one<T, PERIODIC>(a.g, a.ph, a.s, cur, (uint32_t)p, env, box, stats, [&] {
-                if (q < p1) load_parcel(a.s, (uint32_t)q, a.ph.rho, nxt);
-            });
-            cur = nxt;
-        }
-    } else {
-        for (; p < p1; p += blockDim.x) {
-            const int32_t q = p + (int32_t)blockDim.x;
-            step_parcel<T, PERIODIC>(a.g, a.ph, a.s, cur, (uint32_t)p, nsteps, env, box, [&] {
-                if (q < p1) load_parcel(a.s, (uint32_t)q, a.ph.rho, nxt);
-            });
-            cur = nxt;
+    // Software pipeline: the state of the next two parcels of each thread is in flight
+    // while one steps (by Little's law the loads in flight, not the instruction count,
+    // bound a streaming pass).  Vector-memory counters complete in order, and the
+    // compiler places each wait for the worst path into it, so the loop is shaped to
+    // leave it no pessimistic path:
+    //  * the loads are unconditional (indices clamped to the brick's last parcel, a
+    //    valid address once the brick is non-empty) -- a path that skipped them would
+    //    make every iteration wait for all loads in flight, the prefetch included;
+    //  * three register sets rotate (unrolled by three), so no copy of a register with
+    //    a load in flight forces a wait at the end of an iteration;
+    //  * the first loads are waited for before the loop, so the loop's entry path has
+    //    nothing in flight and the waits inside count past earlier parcels' stores
+    //    instead of waiting for them.
+    if (p1 > p0) {
+        __builtin_amdgcn_s_waitcnt(0);
+        auto ahead = [&](int32_t q) { return min((uint32_t)(q + 2 * bs), last); };
+        if (nsteps == 1) {
+            const bool stats = part != nullptr;
+            auto one = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
+                const uint32_t q = ahead(p);
+                step_parcel_one<T, PERIODIC, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, env, box, stats,
+                                                  [&] { load_parcel<T, RHO>(a.s, q, a.ph.rho, fill); });
+                p += bs;
+            };
+            for (;;) {
+                if (p >= p1) break;
+                one(A, C);
+                if (p >= p1) break;
+                one(B, A);
+                if (p >= p1) break;
+                one(C, B);
+            }
+        } else {
+            auto many = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
+                const uint32_t q = ahead(p);
+                step_parcel<T, PERIODIC, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, nsteps, env, box,
+                                              [&] { load_parcel<T, RHO>(a.s, q, a.ph.rho, fill); });
+                p += bs;
+            };
+            for (;;) {
+                if (p >= p1) break;
+                many(A, C);
+                if (p >= p1) break;
+                many(B, A);
+                if (p >= p1) break;
+                many(C, B);
+            }
         }
     }
     __syncthreads();
 
-    // Add the tile's sums to HBM.  With the owner flush only this workgroup touches the
-    // brick's own cells of `src` during the launch (deposits from other workgroups'
-    // halos and strays go to `stray`), so a plain read-modify-write is race-free there;
-    // the halo's sums go to `stray` with atomics.
-    for (int32_t t = threadIdx.x; t < ncell; t += blockDim.x) {
-        const double a0 = St[3 * t], a1 = St[3 * t + 1], a2 = St[3 * t + 2];
-        if (a0 == 0.0 && a1 == 0.0 && a2 == 0.0) continue;
-        const int32_t li = t % wx;
-        const int32_t r = t / wx;
-        const int32_t lj = r % wy;
-        const int32_t lk = r / wy;
-        const int32_t gi = env.h0 + li, gj = env.h1 + lj, gk = env.h2 + lk;
-        const uint32_t c = (uint32_t)gk * a.g.nxny + (uint32_t)gj * (uint32_t)a.g.n[0] + (uint32_t)gi;
-        const bool own = (uint32_t)(li - 1) < (uint32_t)ex && (uint32_t)(lj - 1) < (uint32_t)ey &&
-                         (uint32_t)(lk - 1) < (uint32_t)ez;
-        if (own && a.flush == 0) {
-            double *q = a.src + 3u * c;
-            q[0] += a0;
-            q[1] += a1;
-            q[2] += a2;
-        } else {
-            double *q = env.out + 3u * c;
-            atomic_add_global(q, a0);
-            atomic_add_global(q + 1, a1);
-            atomic_add_global(q + 2, a2);
+    // Add the tile's sums to HBM, kStage cells per thread at a time with every load of
+    // the read-modify-writes issued before the first add.  With the owner flush only
+    // this workgroup touches the brick's own cells of `src` during the launch
+    // (deposits from other workgroups' halos and strays go to `stray`), so a plain
+    // read-modify-write is race-free there; the halo's sums go to `stray` with atomics.
+    // A cell with a nonzero sum lies inside the mesh (deposits go to folded or clamped
+    // cells only).
+    for (int32_t t0 = 0; t0 < ncell; t0 += kStage * bs) {
+        double v[kStage][3], old[kStage][3];
+        uint32_t c[kStage];
+        bool nz[kStage], own[kStage];
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int32_t t = t0 + k * bs + (int32_t)threadIdx.x;
+            const int32_t tc = min(t, ncell - 1);
+            v[k][0] = St[3 * tc];
+            v[k][1] = St[3 * tc + 1];
+            v[k][2] = St[3 * tc + 2];
+            nz[k] = t < ncell && (v[k][0] != 0.0 || v[k][1] != 0.0 || v[k][2] != 0.0);
+            int32_t li, lj, lk;
+            ts.split(tc, li, lj, lk);
+            c[k] = (uint32_t)(env.h2 + lk) * a.g.nxny + (uint32_t)(env.h1 + lj) * (uint32_t)a.g.n[0] +
+                   (uint32_t)(env.h0 + li);
+            own[k] = a.flush == 0 && (uint32_t)(li - 1) < (uint32_t)ex && (uint32_t)(lj - 1) < (uint32_t)ey &&
+                     (uint32_t)(lk - 1) < (uint32_t)ez;
+            if (nz[k] && own[k]) {
+                const double *q = a.src + 3u * c[k];
+                old[k][0] = q[0];
+                old[k][1] = q[1];
+                old[k][2] = q[2];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            if (!nz[k]) continue;
+            if (own[k]) {
+                double *q = a.src + 3u * c[k];
+                q[0] = old[k][0] + v[k][0];
+                q[1] = old[k][1] + v[k][1];
+                q[2] = old[k][2] + v[k][2];
+            } else {
+                double *q = env.out + 3u * c[k];
+                atomic_add_global(q, v[k][0]);
+                atomic_add_global(q + 1, v[k][1]);
+                atomic_add_global(q + 2, v[k][2]);
+            }
         }
     }
     if (part) flush_stats(box, env.strays, part + (size_t)brick * kStatSlot);
 }
 
-template <class T, bool PERIODIC>
+template <class T, bool PERIODIC, bool RHO>
 __global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
     extern __shared__ double lds[];
-    process_brick<T, PERIODIC>(a, xcd_remap(blockIdx.x, a.nbricks), a.nsteps, lds, a.part);
+    // Bricks in dispatch order: workgroups go round-robin over the XCDs, so the bricks
+    // in flight at any time form one contiguous window of the parcel arrays (measured
+    // faster on MI355X than giving each XCD its own contiguous range of bricks, which
+    // spreads the concurrent streams over eight windows).
+    process_brick<T, PERIODIC, RHO>(a, blockIdx.x, a.nsteps, lds, a.part);
 }
 
 // Many clouds in one persistent launch: each workgroup takes (cloud, brick) work items
@@ -457,7 +565,10 @@ __global__ void __launch_bounds__(1024) k_pass_batch(const PassArgs<T> *__restri
         }
         const int32_t c = __builtin_amdgcn_readfirstlane(lo);
         const PassArgs<T> a = descs[c];
-        process_brick<T, PERIODIC>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
+        if (a.s.rho)  // uniform: one cloud per work item
+            process_brick<T, PERIODIC, true>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
+        else
+            process_brick<T, PERIODIC, false>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
         __syncthreads();  // s_item is rewritten only after every thread has read it
     }
 }
@@ -491,20 +602,30 @@ template <class T>
 hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s) {
     if (a.np == 0) return hipSuccess;
     const uint32_t grid = (a.np + block - 1) / block;
-    if (periodic)
-        k_pass_plain<T, true><<<grid, block, 0, s>>>(a);
+    const bool rho = a.s.rho != nullptr;
+    if (periodic && rho)
+        k_pass_plain<T, true, true><<<grid, block, 0, s>>>(a);
+    else if (periodic)
+        k_pass_plain<T, true, false><<<grid, block, 0, s>>>(a);
+    else if (rho)
+        k_pass_plain<T, false, true><<<grid, block, 0, s>>>(a);
     else
-        k_pass_plain<T, false><<<grid, block, 0, s>>>(a);
+        k_pass_plain<T, false, false><<<grid, block, 0, s>>>(a);
     return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_pass_sorted(const PassArgs<T> &a, int periodic, int block, hipStream_t s) {
     const size_t lds = (size_t)a.tile_cells * 3 * (sizeof(double) + sizeof(T));
-    if (periodic)
-        k_pass_sorted<T, true><<<a.nbricks, block, lds, s>>>(a);
+    const bool rho = a.s.rho != nullptr;
+    if (periodic && rho)
+        k_pass_sorted<T, true, true><<<a.nbricks, block, lds, s>>>(a);
+    else if (periodic)
+        k_pass_sorted<T, true, false><<<a.nbricks, block, lds, s>>>(a);
+    else if (rho)
+        k_pass_sorted<T, false, true><<<a.nbricks, block, lds, s>>>(a);
     else
-        k_pass_sorted<T, false><<<a.nbricks, block, lds, s>>>(a);
+        k_pass_sorted<T, false, false><<<a.nbricks, block, lds, s>>>(a);
     return hipGetLastError();
 }
 
