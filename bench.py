@@ -26,6 +26,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "scale-track_amd"), ROOT]
 
 PEAK_HBM_GBS = 8000.0
+# rocprofv3 summary of the headline kernel on the default forward workload: its
+# FETCH_SIZE + WRITE_SIZE traffic per launch (calibrated against a copy of known size)
+# fills roofline.traffic when this run's workload is the one profiled.
+PROFILE = os.path.join(ROOT, "profiles", "k_pass_sorted_fp32_16M_128c.md")
+
+
+def profiled_traffic(path, want):
+    """HBM bytes per launch from a committed rocprofv3 summary, or None when the
+    summary is missing or was measured on another workload."""
+    from scaletrack.rocprof_summary import read_figures
+
+    if not os.path.exists(path):
+        return None
+    fig = read_figures(path)
+    if any(fig.get(k) != str(v) for k, v in want.items()) or "traffic_bytes" not in fig:
+        return None
+    return float(fig["traffic_bytes"])
 
 
 def parse():
@@ -38,7 +55,7 @@ def parse():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--method", choices=["sorted", "plain"], default="sorted")
     ap.add_argument("--substeps-per-pass", type=int, default=1)
-    ap.add_argument("--brick", type=str, default="16,8,4")
+    ap.add_argument("--brick", type=str, default="32,8,2")
     ap.add_argument("--block", type=int, default=512)
     ap.add_argument("--flush", choices=["owner", "atomic"], default="owner")
     ap.add_argument("--only", action="store_true",
@@ -302,7 +319,10 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(PROFILE, {
+                "precision": args.precision, "method": args.method, "parcels": case.np, "cells": args.cells,
+                "brick": ",".join(str(b) for b in brick), "block": args.block,
+                "substeps_per_pass": args.substeps_per_pass, "flush": args.flush}),
             "kernel": "k_pass_sorted" if args.method == "sorted" else "k_pass_plain",
             "kernel_ms_per_pass": head["kernel_ms_per_pass"],
             "bytes_per_pass": bpp,

@@ -78,7 +78,7 @@ typedef struct st_params {
     int32_t flush;             /* st_flush, default ST_FLUSH_OWNER */
     int32_t substeps_per_pass; /* substeps fused in registers per launch; 0 = all nsub */
     double cd;                 /* drag-coefficient multiplier, default 1 (reference drag) */
-    int32_t brick[3];          /* cells per brick (sorted method); default 16,8,4 */
+    int32_t brick[3];          /* cells per brick (sorted method); default 32,8,2 */
     int32_t block;             /* threads per workgroup; default 512 */
     double resort_fraction;    /* re-sort when the share of parcels outside their brick in
                                   the last pass exceeds this; default 0.08 */
@@ -174,7 +174,8 @@ int st_batch_evolve(st_batch *batch, double dt_fluid, int32_t nsub);
 /* One fluid step of the whole batch as a hipGraph (optionally zeroing every UTrans). */
 int st_batch_graph_capture(st_batch *batch, double dt_fluid, int32_t nsub, int32_t zero_source,
                            st_graph **out);
-/* Work items, persistent grid size, and the brick of each cloud (3*n, caller order);
+/* Work items, persistent workgroups per pass (summed over the launches of clouds with
+ * and without a per-parcel density), and the brick of each cloud (3*n, caller order);
  * any output may be NULL. */
 int st_batch_info(const st_batch *batch, int32_t *total_items, int32_t *grid,
                   int32_t *brick_of_cloud);

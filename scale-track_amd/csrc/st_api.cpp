@@ -359,9 +359,9 @@ void st_params_default(st_params *p) {
     p->flush = ST_FLUSH_OWNER;
     p->substeps_per_pass = 0;
     p->cd = 1.0;
-    p->brick[0] = 16;
+    p->brick[0] = 32;
     p->brick[1] = 8;
-    p->brick[2] = 4;
+    p->brick[2] = 2;
     p->block = 512;
     p->resort_fraction = 0.08;
 }

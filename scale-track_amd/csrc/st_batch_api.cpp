@@ -19,7 +19,8 @@ struct st_batch {
     st_params p{};
     int32_t target = 0;
     int32_t blocks_per_cu = 0;
-    std::vector<st_cloud *> clouds;  // in work order (densest first)
+    std::vector<st_cloud *> clouds;  // in work order: no per-parcel density first, then
+                                     // per-parcel density; densest first in each group
     std::vector<st_field *> fields;
     std::vector<st_params> cloud_params;
     std::vector<int> caller_index;   // slot -> position in the caller's arrays
@@ -29,7 +30,9 @@ struct st_batch {
     st::StatsTarget *targets = nullptr;  // device [n]: statistics reduction of each cloud
     unsigned int *counter = nullptr; // device
     int32_t total_items = 0;
-    int grid = 0;
+    int32_t nplain = 0;              // leading clouds without a per-parcel density
+    std::vector<int32_t> host_item0; // [n + 1]
+    int grid[2] = {0, 0};            // persistent grid of each group (0: group empty)
     size_t lds = 0;
     double dt_sub = -1.0;            // substep the device table holds
 };
@@ -96,6 +99,7 @@ int build_table(st_batch *b, double dt_sub) {
         maxcells = std::max(maxcells, a.tile_cells);
     }
     b->total_items = item0[n];
+    b->host_item0 = item0;
     b->lds = (size_t)maxcells * 3 * (sizeof(double) + sizeof(T));
     hipStream_t s = b->ctx->stream;
     ST_HIP(hipMemcpyAsync(b->descs, hd, b->host_descs.size(), hipMemcpyHostToDevice, s));
@@ -105,10 +109,16 @@ int build_table(st_batch *b, double dt_sub) {
     int dev = b->ctx->device;
     int cus = 0;
     ST_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    int per = st::batch_max_blocks_per_cu<T>(b->p.boundary == ST_PERIODIC, b->p.block, b->lds);
-    if (per < 1) return fail(ST_E_RANGE, "batch kernel does not fit: block %d, %zu B LDS", b->p.block, b->lds);
-    if (b->blocks_per_cu > 0) per = std::min(per, b->blocks_per_cu);
-    b->grid = std::max(1, std::min(cus * per, b->total_items));
+    for (int g = 0; g < 2; ++g) {
+        const int32_t c0 = g == 0 ? 0 : b->nplain, c1 = g == 0 ? b->nplain : (int32_t)n;
+        const int32_t items = item0[c1] - item0[c0];
+        b->grid[g] = 0;
+        if (c1 == c0) continue;
+        int per = st::batch_max_blocks_per_cu<T>(b->p.boundary == ST_PERIODIC, g, b->p.block, b->lds);
+        if (per < 1) return fail(ST_E_RANGE, "batch kernel does not fit: block %d, %zu B LDS", b->p.block, b->lds);
+        if (b->blocks_per_cu > 0) per = std::min(per, b->blocks_per_cu);
+        b->grid[g] = std::max(1, std::min(cus * per, items));
+    }
     b->dt_sub = dt_sub;
     return ST_OK;
 }
@@ -130,18 +140,27 @@ int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
     const int per = b->p.substeps_per_pass > 0 ? b->p.substeps_per_pass : nsub;
     if (!capturing) ST_HIP(hipEventRecord(b->ctx->ev[0], s));
     int launches = 0;
+    const int32_t n = (int32_t)b->clouds.size();
+    const int periodic = b->p.boundary == ST_PERIODIC;
     for (int done = 0; done < nsub; done += per) {
         const int ns = std::min(per, nsub - done);
-        ST_HIP(st::launch_fill_u32(b->counter, 0u, 1, s));
-        if (b->prec == ST_FP64)
-            ST_HIP(st::launch_pass_batch<double>((const st::PassArgs<double> *)b->descs, b->item0,
-                                                 (int32_t)b->clouds.size(), ns, done + ns >= nsub, b->counter,
-                                                 b->p.boundary == ST_PERIODIC, b->p.block, b->grid, b->lds, s));
-        else
-            ST_HIP(st::launch_pass_batch<float>((const st::PassArgs<float> *)b->descs, b->item0,
-                                                (int32_t)b->clouds.size(), ns, done + ns >= nsub, b->counter,
-                                                b->p.boundary == ST_PERIODIC, b->p.block, b->grid, b->lds, s));
-        ++launches;
+        const int stats = done + ns >= nsub;
+        // one persistent launch per group of clouds sharing the density form, so each
+        // launch runs the kernel instance of that form
+        for (int g = 0; g < 2; ++g) {
+            if (!b->grid[g]) continue;
+            const int32_t c0 = g == 0 ? 0 : b->nplain, nc = g == 0 ? b->nplain : n - b->nplain;
+            ST_HIP(st::launch_fill_u32(b->counter, (unsigned)b->host_item0[c0], 1, s));
+            if (b->prec == ST_FP64)
+                ST_HIP(st::launch_pass_batch<double>((const st::PassArgs<double> *)b->descs + c0, b->item0 + c0, nc,
+                                                     ns, stats, b->counter, periodic, g, b->p.block, b->grid[g],
+                                                     b->lds, s));
+            else
+                ST_HIP(st::launch_pass_batch<float>((const st::PassArgs<float> *)b->descs + c0, b->item0 + c0, nc,
+                                                    ns, stats, b->counter, periodic, g, b->p.block, b->grid[g],
+                                                    b->lds, s));
+        }
+        ++launches;  // passes: the kernel time per pass spans both launches
     }
     if (!capturing) {
         ST_HIP(hipEventRecord(b->ctx->ev[1], s));
@@ -198,7 +217,13 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
         idx[i] = i;
         density[i] = (double)clouds[i]->n / (double)fields[i]->ncell;
     }
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return density[a] > density[c]; });
+    // clouds without a per-parcel density first, then those with one; densest first in
+    // each group (the longest work items are handed out first)
+    auto has_rho = [&](int i) { return clouds[i]->has_rho != 0; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) {
+        if (has_rho(a) != has_rho(c)) return !has_rho(a);
+        return density[a] > density[c];
+    });
     st_batch *b = new (std::nothrow) st_batch;
     if (!b) return fail(ST_E_NOMEM, "host allocation failed");
     b->ctx = ctx;
@@ -209,6 +234,7 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
     b->blocks_per_cu = blocks_per_cu;
     for (int k = 0; k < n; ++k) {
         const int i = idx[k];
+        if (!has_rho(i)) b->nplain = k + 1;
         b->clouds.push_back(clouds[i]);
         b->fields.push_back(fields[i]);
         b->caller_index.push_back(i);
@@ -318,7 +344,7 @@ int st_batch_graph_capture(st_batch *b, double dt_fluid, int32_t nsub, int32_t z
 int st_batch_info(const st_batch *b, int32_t *total_items, int32_t *grid, int32_t *brick_of_cloud) {
     if (!b) return fail(ST_E_INVALID, "null batch");
     if (total_items) *total_items = b->total_items;
-    if (grid) *grid = b->grid;
+    if (grid) *grid = b->grid[0] + b->grid[1];
     if (brick_of_cloud)  // in the caller's cloud order
         for (size_t i = 0; i < b->clouds.size(); ++i)
             for (int k = 0; k < 3; ++k) brick_of_cloud[3 * b->caller_index[i] + k] = b->cloud_params[i].brick[k];

@@ -131,7 +131,7 @@ inline double substep_dt(int prec, double dt_fluid, int nsub) {
     return prec == ST_FP64 ? dt_fluid / (double)nsub : (double)((float)dt_fluid / (float)nsub);
 }
 template <class T>
-inline Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd);
+Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd);  // st_api.cpp
 int merge_source(st_field *f);
 // ST_E_INVALID unless the brick's halo tile fits in one workgroup's LDS.
 int check_tile(const st_ctx *ctx, int prec, const int32_t brick[3]);

@@ -101,12 +101,15 @@ template <class T>
 hipError_t launch_pass_sorted(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
 template <class T>
 hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
+// Persistent pass over clouds [0, ncloud) of descs: work items item0[0] .. item0[ncloud]-1
+// (counter must hold item0[0] at launch); rho: every one of these clouds carries a
+// per-parcel density array (else none does).
 template <class T>
 hipError_t launch_pass_batch(const PassArgs<T> *descs, const int32_t *item0, int32_t ncloud, int32_t nsteps,
-                             int stats, unsigned int *counter, int periodic, int block, int grid, size_t lds,
-                             hipStream_t s);
+                             int stats, unsigned int *counter, int periodic, int rho, int block, int grid,
+                             size_t lds, hipStream_t s);
 template <class T>
-int batch_max_blocks_per_cu(int periodic, int block, size_t lds);
+int batch_max_blocks_per_cu(int periodic, int rho, int block, size_t lds);
 hipError_t launch_reduce_stats(const StatsTarget &t, hipStream_t s);
 hipError_t launch_reduce_stats_many(const StatsTarget *targets, int32_t n, hipStream_t s);
 template <class T>

@@ -376,13 +376,10 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     const int32_t bs = (int32_t)blockDim.x;
     const uint32_t last = (uint32_t)(p1 - 1);
     int32_t p = p0 + (int32_t)threadIdx.x;
-    // The first parcels' state is requested before the tile is staged, so its latency
+    // The first parcel's state is requested before the tile is staged, so its latency
     // overlaps the staging (the brick is non-empty: a clamped index is valid).
-    ParcelIn<T> A, B, C;
-    if (p1 > p0) {
-        load_parcel<T, RHO>(a.s, min((uint32_t)p, last), a.ph.rho, A);
-        load_parcel<T, RHO>(a.s, min((uint32_t)(p + bs), last), a.ph.rho, B);
-    }
+    ParcelIn<T> A, B;
+    if (p1 > p0) load_parcel<T, RHO>(a.s, min((uint32_t)p, last), a.ph.rho, A);
 
     // Stage the tile's velocities and clear its sums, kStage cells per thread at a
     // time with every load issued before the first LDS write (one memory round trip per
@@ -427,22 +424,22 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
 
     Box<T> box;
     box.init();
-    // Software pipeline: the state of the next two parcels of each thread is in flight
-    // while one steps (by Little's law the loads in flight, not the instruction count,
-    // bound a streaming pass).  Vector-memory counters complete in order, and the
-    // compiler places each wait for the worst path into it, so the loop is shaped to
-    // leave it no pessimistic path:
+    // Software pipeline: the next parcel's state is in flight while one steps.
+    // Vector-memory counters complete in order, and the compiler places each wait for
+    // the worst path into it, so the loop is shaped to leave it no pessimistic path:
     //  * the loads are unconditional (indices clamped to the brick's last parcel, a
     //    valid address once the brick is non-empty) -- a path that skipped them would
     //    make every iteration wait for all loads in flight, the prefetch included;
-    //  * three register sets rotate (unrolled by three), so no copy of a register with
-    //    a load in flight forces a wait at the end of an iteration;
+    //  * two register sets alternate (unrolled by two), so no copy of a register with a
+    //    load in flight forces a wait at the end of an iteration;
     //  * the first loads are waited for before the loop, so the loop's entry path has
     //    nothing in flight and the waits inside count past earlier parcels' stores
     //    instead of waiting for them.
+    // (A second parcel in flight per thread measured no faster and costs the FP64 and
+    // batch kernels occupancy.)
     if (p1 > p0) {
         __builtin_amdgcn_s_waitcnt(0);
-        auto ahead = [&](int32_t q) { return min((uint32_t)(q + 2 * bs), last); };
+        auto ahead = [&](int32_t q) { return min((uint32_t)(q + bs), last); };
         if (nsteps == 1) {
             const bool stats = part != nullptr;
             auto one = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
@@ -453,11 +450,9 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             };
             for (;;) {
                 if (p >= p1) break;
-                one(A, C);
+                one(A, B);
                 if (p >= p1) break;
                 one(B, A);
-                if (p >= p1) break;
-                one(C, B);
             }
         } else {
             auto many = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
@@ -468,11 +463,9 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             };
             for (;;) {
                 if (p >= p1) break;
-                many(A, C);
+                many(A, B);
                 if (p >= p1) break;
                 many(B, A);
-                if (p >= p1) break;
-                many(C, B);
             }
         }
     }
@@ -543,7 +536,7 @@ __global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
 // from a global counter until none are left.  item0[c] is the first item of cloud c
 // (clouds ordered densest first, so the longest items are handed out first); every
 // workgroup leaves once the counter passes item0[ncloud].
-template <class T, bool PERIODIC>
+template <class T, bool PERIODIC, bool RHO>
 __global__ void __launch_bounds__(1024) k_pass_batch(const PassArgs<T> *__restrict__ descs,
                                                      const int32_t *__restrict__ item0, int32_t ncloud,
                                                      int32_t nsteps, int stats, unsigned int *counter) {
@@ -565,38 +558,46 @@ __global__ void __launch_bounds__(1024) k_pass_batch(const PassArgs<T> *__restri
         }
         const int32_t c = __builtin_amdgcn_readfirstlane(lo);
         const PassArgs<T> a = descs[c];
-        if (a.s.rho)  // uniform: one cloud per work item
-            process_brick<T, PERIODIC, true>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
-        else
-            process_brick<T, PERIODIC, false>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
+        process_brick<T, PERIODIC, RHO>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
         __syncthreads();  // s_item is rewritten only after every thread has read it
     }
 }
 
 template <class T>
 hipError_t launch_pass_batch(const PassArgs<T> *descs, const int32_t *item0, int32_t ncloud, int32_t nsteps,
-                             int stats, unsigned int *counter, int periodic, int block, int grid, size_t lds,
-                             hipStream_t s) {
-    if (periodic)
-        k_pass_batch<T, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
+                             int stats, unsigned int *counter, int periodic, int rho, int block, int grid,
+                             size_t lds, hipStream_t s) {
+    if (periodic && rho)
+        k_pass_batch<T, true, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
+    else if (periodic)
+        k_pass_batch<T, true, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
+    else if (rho)
+        k_pass_batch<T, false, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
     else
-        k_pass_batch<T, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
+        k_pass_batch<T, false, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
     return hipGetLastError();
 }
 template hipError_t launch_pass_batch<float>(const PassArgs<float> *, const int32_t *, int32_t, int32_t, int,
-                                             unsigned int *, int, int, int, size_t, hipStream_t);
+                                             unsigned int *, int, int, int, int, size_t, hipStream_t);
 template hipError_t launch_pass_batch<double>(const PassArgs<double> *, const int32_t *, int32_t, int32_t, int,
-                                              unsigned int *, int, int, int, size_t, hipStream_t);
+                                              unsigned int *, int, int, int, int, size_t, hipStream_t);
 
 template <class T>
-int batch_max_blocks_per_cu(int periodic, int block, size_t lds) {
+int batch_max_blocks_per_cu(int periodic, int rho, int block, size_t lds) {
     int n = 0;
-    hipError_t e = periodic ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_batch<T, true>, block, lds)
-                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_batch<T, false>, block, lds);
+    hipError_t e;
+    if (periodic && rho)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_batch<T, true, true>, block, lds);
+    else if (periodic)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_batch<T, true, false>, block, lds);
+    else if (rho)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_batch<T, false, true>, block, lds);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_batch<T, false, false>, block, lds);
     return e == hipSuccess ? n : 0;
 }
-template int batch_max_blocks_per_cu<float>(int, int, size_t);
-template int batch_max_blocks_per_cu<double>(int, int, size_t);
+template int batch_max_blocks_per_cu<float>(int, int, int, size_t);
+template int batch_max_blocks_per_cu<double>(int, int, int, size_t);
 
 template <class T>
 hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s) {

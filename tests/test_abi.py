@@ -34,7 +34,7 @@ def test_version_and_defaults():
     assert _lib.lib.st_version() == 1
     p = _lib.default_params()
     assert (p.boundary, p.method, p.flush, p.substeps_per_pass) == (0, 1, 0, 0)
-    assert p.cd == 1.0 and list(p.brick) == [16, 8, 4] and p.block == 512
+    assert p.cd == 1.0 and list(p.brick) == [32, 8, 2] and p.block == 512
     assert 0 < p.resort_fraction < 1
 
 
