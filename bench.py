@@ -116,6 +116,71 @@ def run_variant(stk, ctx, case, precision, p, steps, warmup, keep_state=False):
     return out
 
 
+def precision_check(stk, ctx, case, p, calls=5, block=4):
+    """FP32 form against the FP64 path on the same inputs and schedule (calls fluid
+    steps of case.nsub substeps, source zeroed per call).  Both runs start from the
+    FP32-representable inputs (state and cell velocities rounded to FP32), so every
+    difference comes from the form's arithmetic.  Reported per call:
+      * source relative L2 over cells, and max over cells against the largest |S|;
+      * the same on block^3-cell blocks (a parcel that took the other side of a face
+        moves its deposit to a neighbouring cell: blocks sum most such moves back);
+      * total momentum source (sum over cells) relative to the sum of |S|;
+    and at the end: position error in cell widths, fraction of parcels whose final
+    cell differs, velocity relative L2."""
+    runs = {}
+    u32 = case.U.astype(np.float32)
+    st32 = [a.astype(np.float32) for a in case.state()]
+    for prec in (stk.FP64, stk.FP32):
+        dt = np.float32 if prec == stk.FP32 else np.float64
+        mesh = stk.Mesh(case.n, case.origin, case.ending)
+        field = stk.TwoWayEulerian(ctx, mesh, prec)
+        field.upload_U(u32.astype(dt))
+        chunk = stk.Chunk(ctx, case.np, case.mu_c, case.rho_d, case.rho_d_by_rho_c, prec)
+        chunk.set_state(*[a.astype(dt) for a in st32])
+        chunk.upload()
+        srcs = []
+        for _ in range(calls):
+            field.zero_source()
+            stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
+            srcs.append(field.download_source().astype(np.float64))
+        runs[prec] = (srcs, [a.astype(np.float64) for a in chunk.download()])
+        field.close()
+        chunk.close()
+    (s64, st64), (s32, stf) = runs[stk.FP64], runs[stk.FP32]
+    nx, ny, nz = case.n
+
+    def coarse(s):
+        g = s.reshape(nz, ny, nx, 3)
+        b = block
+        return g.reshape(nz // b, b, ny // b, b, nx // b, b, 3).sum(axis=(1, 3, 5))
+
+    out = {"substeps": calls * case.nsub, "block_cells": block, "per_call": []}
+    for a, b in zip(s32, s64):
+        ca, cb = coarse(a), coarse(b)
+        out["per_call"].append({
+            "source_rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+            "source_rel_max": float(np.abs(a - b).max() / np.abs(b).max()),
+            "block_source_rel_l2": float(np.linalg.norm(ca - cb) / np.linalg.norm(cb)),
+            "total_momentum_rel": float(np.abs(a.sum(0) - b.sum(0)).max() / np.abs(b).sum()),
+        })
+    delta = np.array([(e - o) / n for o, e, n in zip(case.origin, case.ending, case.n)])
+    length = np.array([e - o for o, e in zip(case.origin, case.ending)])
+    pos, pos64 = np.stack(stf[1:4], 1), np.stack(st64[1:4], 1)
+    dpos = pos - pos64
+    dpos -= length * np.round(dpos / length)  # a wrap taken in one run only is not an error
+    cells = np.floor((pos - np.asarray(case.origin)) / delta)
+    cells64 = np.floor((pos64 - np.asarray(case.origin)) / delta)
+    vel, vel64 = np.stack(stf[4:7], 1), np.stack(st64[4:7], 1)
+    out.update({
+        "source_rel_l2_max": max(c["source_rel_l2"] for c in out["per_call"]),
+        "block_source_rel_l2_max": max(c["block_source_rel_l2"] for c in out["per_call"]),
+        "position_max_err_cells": float((np.abs(dpos) / delta).max()),
+        "parcels_in_other_cell": float(np.any(cells != cells64, axis=1).mean()),
+        "velocity_rel_l2": float(np.linalg.norm(vel - vel64) / np.linalg.norm(vel64)),
+    })
+    return out
+
+
 def batch_workload(stk, inputs, ctx, nclouds=64, steps=5, warmup=1, seed=19891, tune_cache=None):
     """64 independent periodic clouds, parcel counts log-uniform in [1e3, 2e6], meshes
     n^3 with n uniform in {32..96}, Taylor-Green with seeded random phases, FP32 state;
@@ -352,6 +417,8 @@ def main():
         line["variants"]["fp32_sorted_fused10"]["roofline_frac"] = (
             bytes_per_pass(case.np, nc, True) / (fused32["kernel_ms_per_pass"] * 1e-3) / 1e9 / PEAK_HBM_GBS)
         line["speedup_vs_fp64_plain"] = head["updates_per_s"] / plain64["updates_per_s"]
+        line["fp32_vs_fp64"] = precision_check(stk, ctx, case, ps, calls=args.steps)
+        print(f"# fp32 vs fp64: {json.dumps(line['fp32_vs_fp64'])}", file=sys.stderr, flush=True)
     if not args.only:
         line["batch"] = batch_workload(stk, inputs, ctx, tune_cache=args.tune_cache)
         print(f"# batch: {json.dumps(line['batch'])}", file=sys.stderr, flush=True)

@@ -269,6 +269,42 @@ def test_fp32_form_close_to_fp64():
     assert rel < 1e-2
 
 
+def _as_f64_case(c32):
+    """The FP32 case's (FP32-representable) inputs as a float64 case."""
+    st = [a.astype(np.float64) for a in c32.state()]
+    return inputs.Case(c32.n, c32.origin, c32.ending, c32.U.astype(np.float64), *st, mu_c=c32.mu_c,
+                       rho_d=c32.rho_d, rho_c=c32.rho_c, dt_fluid=c32.dt_fluid, nsub=c32.nsub,
+                       boundary=c32.boundary)
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_fp32_form_arithmetic_error_under_frozen_decisions(boundary):
+    """Accuracy of the FP32 form's arithmetic, separated from its face decisions.
+
+    With a cell-constant carrier velocity the per-cell source is a discontinuous
+    function of the parcel positions: a parcel whose FP32 position lands on the other
+    side of a face than the FP64 one deposits into another cell, which alone moves the
+    per-cell relative L2 error by ~1/(n sqrt(Ncells)) at n parcels per cell.  Here the
+    FP32 run records its decisions (cell, bounce, wrap per substep) and the FP64 run
+    replays them from the same FP32-representable inputs, over one call of 50 substeps
+    of 1e-4 s (a call starts by locating each parcel from its position, so one call
+    keeps every decision frozen): the remaining difference is the FP32 arithmetic
+    (state rounding, FP32 drag and advance) and must stay below 1e-5 relative per cell.
+    The free-running pair differs by far more: that error is the decisions'."""
+    c32 = inputs.taylor_green_box(np_parcels=100000, ncell=24, boundary=boundary, dtype=np.float32,
+                                  dt_fluid=5e-3, nsub=50)
+    c64 = _as_f64_case(c32)
+    ev = np.zeros((c32.np, c32.nsub), dtype=np.int64)
+    s32, _, _ = st_oracle.evolve_case(c32, precision=st_oracle.FP32, events=ev, events_mode=1)
+    s64, _, _ = st_oracle.evolve_case(c64, events=ev, events_mode=2)
+    sfree, _, _ = st_oracle.evolve_case(c64)
+    frozen = np.linalg.norm(s32 - s64) / np.linalg.norm(s64)
+    free = np.linalg.norm(s32 - sfree) / np.linalg.norm(sfree)
+    assert frozen < 1e-5, frozen
+    assert np.abs(s32 - s64).max() < 1e-5 * np.abs(s64).max()
+    assert free > 10 * frozen
+
+
 def test_extrapolator_closed_form():
     """estimate_source! (asyncParallelTracking.jl:738-747): est_n = 2 S_n - est_{n-1}."""
     rng = np.random.default_rng(0)
