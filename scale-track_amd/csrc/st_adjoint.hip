@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
         for (; t <= t0; ++t) {
             double nx, ny, nz, nvx, nvy, nvz;
             Cell nc;
-            advance<double, PERIODIC>(a.g, a.ph, f, den, ux, uy, uz, px, py, pz, vx, vy, vz, cell, part0, nx,
+            advance<double, PERIODIC>(a.g, a.ph, f, Quot<double>(den), ux, uy, uz, px, py, pz, vx, vy, vz, cell, part0, nx,
                                       ny, nz, nvx, nvy, nvz, nc);
             if (t != a.K && nc.c != cell.c) {
                 cell = nc;
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
                 rc[j] = cell.c;
                 double nx, ny, nz, nvx, nvy, nvz;
                 Cell nc;
-                const unsigned bnc = advance<double, PERIODIC>(a.g, a.ph, f, den, ux, uy, uz, px, py, pz, vx, vy,
+                const unsigned bnc = advance<double, PERIODIC>(a.g, a.ph, f, Quot<double>(den), ux, uy, uz, px, py, pz, vx, vy,
                                                                vz, cell, part0, nx, ny, nz, nvx, nvy, nvz, nc);
                 rb |= bnc << (3 * j);
                 if (t != a.K && nc.c != cell.c) {

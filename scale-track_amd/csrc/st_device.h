@@ -129,20 +129,51 @@ __device__ __forceinline__ void load_parcel(const Soa<T> &s, uint32_t p, T rho_c
     in.rho = RHO ? ld_off(s.rho, off) : rho_chunk;
 }
 
+// Division by a parcel's implicit-Euler denominator 1 + f (:1295), correctly rounded in
+// both precision forms.  Three velocity components per substep share the denominator.
+template <class T>
+struct Quot;
+template <>
+struct Quot<double> {
+    double den;
+    __device__ __forceinline__ explicit Quot(double d) : den(d) {}
+    __device__ __forceinline__ double operator()(double a) const { return a / den; }
+};
+// For floats, a / den == (float)((double)a * RN64(1 / den)) whenever the quotient is
+// zero or outside the FP32 subnormal range.  The double product is within 2^-52
+// (relative) of a / den.  With a = A 2^p and den = D 2^q (A, D odd, below 2^24) and
+// a float rounding midpoint m = M 2^r (M odd, 25 bits), A 2^p - M D 2^(q+r) is a
+// nonzero multiple of 2^min(p, q+r) unless a / den == m, which would need the odd A to
+// equal M D >= M > 2^24; so a / den is never a midpoint and never within 2^-49
+// (relative) of one, and the product rounds to the same float.  Midpoints of the
+// subnormal range (M may be small there) take the IEEE float division.  One double
+// reciprocal per parcel then replaces three float divisions per substep.
+template <>
+struct Quot<float> {
+    float den;
+    double r;
+    __device__ __forceinline__ explicit Quot(float d) : den(d), r(1.0 / (double)d) {}
+    __device__ __forceinline__ float operator()(float a) const {
+        const double q = (double)a * r;
+        if (q != 0.0 && fabs(q) < 0x1p-126) return a / den;
+        return (float)q;
+    }
+};
+
 // Kinematics of one substep (asyncParallelTracking.jl:1294-1335): implicit-Euler
 // drag update and advance, relocation, then the specular wall bounce -- only when the
 // folded cell or partition changed -- or the periodic wrap.  On return (n*) is the new
 // state, nc the cell after relocation, and the result has bit a set for each axis a
 // that bounced (the caller applies the source correction s0_a -= 2 v_a of :734).
 template <class T, bool PERIODIC>
-__device__ __forceinline__ unsigned advance(const Geo &g, const Phys<T> &ph, T f, T den, T ux, T uy,
+__device__ __forceinline__ unsigned advance(const Geo &g, const Phys<T> &ph, T f, const Quot<T> &den, T ux, T uy,
                                             T uz, T px, T py, T pz, T vx, T vy, T vz, const Cell &cell,
                                             int32_t part0, T &nx, T &ny, T &nz, T &nvx, T &nvy, T &nvz,
                                             Cell &nc) {
     // :1295-1296
-    nvx = (vx + f * ux) / den;
-    nvy = (vy + f * uy) / den;
-    nvz = (vz + f * uz) / den;
+    nvx = den(vx + f * ux);
+    nvy = den(vy + f * uy);
+    nvz = den(vz + f * uz);
     nx = px + nvx * ph.dt;
     ny = py + nvy * ph.dt;
     nz = pz + nvz * ph.dt;

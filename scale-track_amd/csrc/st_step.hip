@@ -142,6 +142,7 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     double m;
     T f, den;
     parcel_consts(ph, RHO, in.d, in.rho, m, f, den);
+    const Quot<T> q(den);
 
     T px = in.x, py = in.y, pz = in.z;
     T vx = in.u, vy = in.v, vz = in.w;
@@ -159,7 +160,7 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     for (int32_t t = 1; t <= nsteps; ++t) {
         T nx, ny, nz, nvx, nvy, nvz;
         Cell nc;
-        const unsigned b = advance<T, PERIODIC>(g, ph, f, den, ux, uy, uz, px, py, pz, vx, vy, vz, cell,
+        const unsigned b = advance<T, PERIODIC>(g, ph, f, q, ux, uy, uz, px, py, pz, vx, vy, vz, cell,
                                                 part0, nx, ny, nz, nvx, nvy, nvz, nc);
         if (b & 1u) s0x = s0x - 2.0 * (double)vx;  // update_at_boundary!, :730-736
         if (b & 2u) s0y = s0y - 2.0 * (double)vy;
@@ -215,9 +216,10 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
     prefetch();
 
     // :1295-1296
-    T nvx = (vx + f * ux) / den;
-    T nvy = (vy + f * uy) / den;
-    T nvz = (vz + f * uz) / den;
+    const Quot<T> q(den);
+    T nvx = q(vx + f * ux);
+    T nvy = q(vy + f * uy);
+    T nvz = q(vz + f * uz);
     T nx = px + nvx * ph.dt;
     T ny = py + nvy * ph.dt;
     T nz = pz + nvz * ph.dt;
