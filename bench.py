@@ -186,14 +186,11 @@ def batch_workload(stk, inputs, ctx, nclouds=64, steps=5, warmup=1, seed=19891, 
     n^3 with n uniform in {32..96}, Taylor-Green with seeded random phases, FP32 state;
     one persistent launch per substep, the fluid step captured as a hipGraph, brick and
     block sizes from the autotuner."""
-    u = inputs.uniform(seed, 100, 5 * nclouds).reshape(nclouds, 5)
-    counts = np.rint(np.exp(np.log(1e3) + u[:, 0] * (np.log(2e6) - np.log(1e3)))).astype(np.int64)
-    cells = (32 + np.floor(u[:, 1] * 65)).astype(np.int64)
+    cases = inputs.batch_cases(nclouds, seed=seed, dtype=np.float32)
+    counts = np.array([c.np for c in cases], dtype=np.int64)
+    cells = np.array([c.n[0] for c in cases], dtype=np.int64)
     chunks, fields, bytes_pass = [], [], 0
-    for i in range(nclouds):
-        c = inputs.taylor_green_box(np_parcels=int(counts[i]), ncell=int(cells[i]), dtype=np.float32,
-                                    boundary=stk.PERIODIC, seed=seed + 1000 + i)
-        c.U = inputs.taylor_green(c.n, c.origin, c.ending, phase=tuple(2 * np.pi * u[i, 2:5]), dtype=np.float32)
+    for c in cases:
         field = stk.TwoWayEulerian(ctx, stk.Mesh(c.n, c.origin, c.ending), stk.FP32)
         field.upload_U(c.U)
         chunk = stk.Chunk(ctx, c.np, c.mu_c, c.rho_d, c.rho_d_by_rho_c, stk.FP32)
@@ -245,25 +242,21 @@ def adjoint_workload(stk, inputs, ctx, np_parcels=1 << 20, ncell=64, nsub=200, K
     of the deposited source against a seeded target; gradients for d, rho and cd."""
     from scaletrack.adjoint import Adjoint
 
-    n = (ncell,) * 3
-    o, e = (0.0,) * 3, (1.0,) * 3
-    U = inputs.fourier_field(n, o, e, modes=8, seed=seed)
-    x, y, z = inputs.positions(seed, np_parcels, o, e)
-    d = inputs.lognormal_diameters(seed, np_parcels)
-    rho = 1000.0 * (1.0 + 0.5 * inputs.uniform(seed, 50, np_parcels))
-    zero = np.zeros(np_parcels)
-    field = stk.TwoWayEulerian(ctx, stk.Mesh(n, o, e), stk.FP64)
-    field.upload_U(U)
-    chunk = stk.Chunk(ctx, np_parcels, 1e-3, 1000.0, 1.0, stk.FP64, rho_p=rho, rho_c=1000.0)
-    chunk.set_state(d, x, y, z, zero, zero, zero)
+    case = inputs.adjoint_case(np_parcels, ncell, nsub, seed=seed)
+    n = case.n
+    field = stk.TwoWayEulerian(ctx, stk.Mesh(case.n, case.origin, case.ending), stk.FP64)
+    field.upload_U(case.U)
+    chunk = stk.Chunk(ctx, np_parcels, case.mu_c, case.rho_d, case.rho_d_by_rho_c, stk.FP64, rho_p=case.rho,
+                      rho_c=case.rho_c)
+    chunk.set_state(*case.state())
     chunk.upload()
-    T = 1e-12 * inputs.normal(seed, 60, ncell ** 3 * 3).reshape(-1, 3)
+    T = inputs.target_source(case, seed=seed)
     p = stk.params(boundary=stk.PERIODIC)
     adj = Adjoint(ctx, chunk, field, nsub, K, p)
-    dt_fluid = nsub * 1e-4
+    dt_fluid = case.dt_fluid
 
     def run():
-        chunk.set_state(d, x, y, z, zero, zero, zero)
+        chunk.set_state(*case.state())
         chunk.upload()
         t0 = time.perf_counter()
         adj.forward(dt_fluid)

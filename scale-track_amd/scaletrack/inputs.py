@@ -196,3 +196,40 @@ def cavity_case(np_parcels=10000, dtype=np.float64, seed=SEED, name="2D lid-driv
     mu_c = 0.01 * rho_c  # nu = 0.01 m^2/s
     return Case(n, o, e, U, d, x, y, z, zero, zero.copy(), zero.copy(), mu_c=mu_c, rho_d=2500.0,
                 rho_c=rho_c, dt_fluid=0.005, nsub=10, boundary=0, name=name)
+
+
+def batch_cases(nclouds=64, seed=SEED, dtype=np.float32):
+    """Many independent periodic clouds: parcel counts log-uniform in [1e3, 2e6], meshes
+    n^3 with n uniform in {32..96}, Taylor-Green flow with seeded random phases per cloud
+    (the batched workload)."""
+    u = uniform(seed, 100, 5 * nclouds).reshape(nclouds, 5)
+    counts = np.rint(np.exp(np.log(1e3) + u[:, 0] * (np.log(2e6) - np.log(1e3)))).astype(np.int64)
+    cells = (32 + np.floor(u[:, 1] * 65)).astype(np.int64)
+    cases = []
+    for i in range(nclouds):
+        c = taylor_green_box(np_parcels=int(counts[i]), ncell=int(cells[i]), dtype=dtype, boundary=1,
+                             seed=seed + 1000 + i)
+        c.U = taylor_green(c.n, c.origin, c.ending, phase=tuple(2 * np.pi * u[i, 2:5]), dtype=dtype)
+        cases.append(c)
+    return cases
+
+
+def adjoint_case(np_parcels=1 << 20, ncell=64, nsub=200, seed=SEED, boundary=1):
+    """The adjoint workload: ncell^3 cells on [0,1]^3 with a frozen divergence-free field
+    of 8 seeded Fourier modes; parcels at uniform positions, log-normal diameters, per-
+    parcel density 1000 (1 + U[0, 0.5)) kg/m^3, starting at rest; nsub substeps of
+    1e-4 s in one call.  The loss target is target_source(case, seed)."""
+    n = (ncell,) * 3
+    o, e = (0.0,) * 3, (1.0,) * 3
+    U = fourier_field(n, o, e, modes=8, seed=seed)
+    x, y, z = positions(seed, np_parcels, o, e)
+    d = lognormal_diameters(seed, np_parcels)
+    rho = 1000.0 * (1.0 + 0.5 * uniform(seed, 50, np_parcels))
+    zero = np.zeros(np_parcels)
+    return Case(n, o, e, U, d, x, y, z, zero, zero.copy(), zero.copy(), rho=rho, mu_c=1e-3, rho_d=1000.0,
+                rho_c=1000.0, dt_fluid=nsub * 1e-4, nsub=nsub, boundary=boundary, name="adjoint")
+
+
+def target_source(case, seed=SEED, scale=1e-12):
+    """Seeded target field of the source-misfit loss L = |S - T|^2."""
+    return scale * normal(seed, 60, case.ncell * 3).reshape(case.ncell, 3)
