@@ -90,16 +90,16 @@ def run_variant(stk, ctx, case, precision, p, steps, warmup, keep_state=False):
         if keep_state:
             srcs.append(field.download_source().copy())
     ctx.synchronize()
-    kern_ms, launches = [], 0
     t0 = time.perf_counter()
     for _ in range(steps):
         field.zero_source()
         stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
-        ms, n = ctx.last_kernel_ms()
-        kern_ms.append(ms)
-        launches = n
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
+    # pass durations of the timed steps, from the HIP events around their pass launches
+    timed = ctx.kernel_times(steps)
+    kern_ms = [ms for ms, _ in timed]
+    launches = timed[-1][1]
     out = {
         "elapsed_s": elapsed,
         "ms_per_step": 1e3 * elapsed / steps,
@@ -114,6 +114,30 @@ def run_variant(stk, ctx, case, precision, p, steps, warmup, keep_state=False):
     field.close()
     chunk.close()
     return out
+
+
+def graph_variant(stk, ctx, case, precision, p, steps, warmup):
+    """The same steps replayed from one captured hipGraph of a fluid step (UTrans zeroed,
+    nsub passes): wall-clock parcel-updates/s without per-launch host work."""
+    dt = np.float32 if precision == stk.FP32 else np.float64
+    field = stk.TwoWayEulerian(ctx, stk.Mesh(case.n, case.origin, case.ending), precision)
+    field.upload_U(case.U.astype(dt))
+    chunk = stk.Chunk(ctx, case.np, case.mu_c, case.rho_d, case.rho_d_by_rho_c, precision)
+    chunk.set_state(*[a.astype(dt) for a in case.state()])
+    chunk.upload()
+    graph = stk.Graph(ctx, chunk, field, case.dt_fluid, case.nsub, p, zero_source=True)
+    for _ in range(warmup):
+        graph.launch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.launch()
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    graph.close()
+    field.close()
+    chunk.close()
+    return {"updates_per_s": case.np * case.nsub * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps}
 
 
 def precision_check(stk, ctx, case, p, calls=5, block=4):
@@ -410,6 +434,7 @@ def main():
         line["variants"]["fp32_sorted_fused10"]["roofline_frac"] = (
             bytes_per_pass(case.np, nc, True) / (fused32["kernel_ms_per_pass"] * 1e-3) / 1e9 / PEAK_HBM_GBS)
         line["speedup_vs_fp64_plain"] = head["updates_per_s"] / plain64["updates_per_s"]
+        line["variants"]["fp32_sorted_graph"] = graph_variant(stk, ctx, case, precision, p, args.steps, args.warmup)
         line["fp32_vs_fp64"] = precision_check(stk, ctx, case, ps, calls=args.steps)
         print(f"# fp32 vs fp64: {json.dumps(line['fp32_vs_fp64'])}", file=sys.stderr, flush=True)
     if not args.only:

@@ -228,6 +228,11 @@ int st_adjoint_gradients(st_adjoint *adj, double *grad_d, double *grad_rho, doub
  * copy bandwidth the parcel step's HBM figures are compared with, and the known-size
  * kernel that hardware traffic counters are calibrated on. */
 int st_diag_copy_bandwidth(st_ctx *ctx, int64_t bytes, int32_t reps, double *gbs);
+/* Per-launch device time (ms) of the dominant kernels of each of the last timed calls
+ * (st_evolve, st_batch_evolve, st_adjoint_backward, st_diag_copy_bandwidth), oldest
+ * first: up to min(max, 64) calls, their count in *count, launches per call in
+ * launches[].  Lets a caller time consecutive calls with one synchronisation. */
+int st_diag_kernel_times(st_ctx *ctx, int32_t max, double *ms, int32_t *launches, int32_t *count);
 
 #ifdef __cplusplus
 }

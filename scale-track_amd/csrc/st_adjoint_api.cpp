@@ -198,14 +198,13 @@ int st_adjoint_backward(st_adjoint *a) {
     h.ph = make_phys<double>(c, f, a->dt_sub, a->p.cd);
     h.np = n;
     h.periodic = a->p.boundary == ST_PERIODIC;
-    ST_HIP(hipEventRecord(a->ctx->ev[0], s));
+    ST_HIP(timer_start(a->ctx, s));
     for (int32_t seg = a->nseg - 1; seg >= 0; --seg) {
         h.ck = a->ck + (size_t)seg * 6 * n;
         h.K = std::min(a->K, a->nsub - seg * a->K);
         ST_HIP(st::launch_adjoint_segment(h, s));
     }
-    ST_HIP(hipEventRecord(a->ctx->ev[1], s));
-    a->ctx->timed_launches = a->nseg;
+    ST_HIP(timer_stop(a->ctx, s, a->nseg));
     ST_HIP(st::launch_fill_f64(a->scal + 1, 0.0, 1, s));
     // gd -> tmp[0, n), grho -> tmp[n, 2n), device order
     ST_HIP(st::launch_adjoint_grads(cur.d, cur.rho, a->fbar, a->mbar, a->tmp, cur.rho ? a->tmp + n : nullptr,

@@ -6,6 +6,7 @@
 // bounding-box readback (:1106, :1113).  No status escapes as an exception.
 #include "st_handles.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -42,11 +43,26 @@ void retain_ctx(st_ctx *ctx) {
     if (ctx) ++ctx->refs;
 }
 
+hipError_t timer_start(st_ctx *ctx, hipStream_t s) {
+    return hipEventRecord(ctx->tev[ctx->tcount % kTimerRing][0], s);
+}
+
+hipError_t timer_stop(st_ctx *ctx, hipStream_t s, int launches) {
+    const int slot = (int)(ctx->tcount % kTimerRing);
+    const hipError_t e = hipEventRecord(ctx->tev[slot][1], s);
+    if (e == hipSuccess) {
+        ctx->tlaunches[slot] = launches;
+        ++ctx->tcount;
+    }
+    return e;
+}
+
 static void free_ctx(st_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->ev[0]) (void)hipEventDestroy(ctx->ev[0]);
-    if (ctx->ev[1]) (void)hipEventDestroy(ctx->ev[1]);
+    for (int k = 0; k < kTimerRing; ++k)
+        for (int j = 0; j < 2; ++j)
+            if (ctx->tev[k][j]) (void)hipEventDestroy(ctx->tev[k][j]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -237,7 +253,7 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     // Statistics (bounding box, strays) are taken in the last pass only.
     const int32_t nslots = p.method == ST_METHOD_SORTED ? c->nbricks : (int32_t)((c->n + p.block - 1) / p.block);
     ST_TRY(ensure_part(c, nslots));
-    if (!capturing) ST_HIP(hipEventRecord(ctx->ev[0], s));
+    if (!capturing) ST_HIP(timer_start(ctx, s));
     int launches = 0;
     for (int done = 0; done < nsub; done += per) {
         a.nsteps = per < nsub - done ? per : nsub - done;
@@ -248,10 +264,7 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
             ST_HIP(launch_pass_plain<T>(a, periodic, p.block, s));
         ++launches;
     }
-    if (!capturing) {
-        ST_HIP(hipEventRecord(ctx->ev[1], s));
-        ctx->timed_launches = launches;
-    }
+    if (!capturing) ST_HIP(timer_stop(ctx, s, launches));
     ST_HIP(launch_reduce_stats(StatsTarget{c->part, nslots, c->bbox, c->strays}, s));
     if (p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER) f->stray_dirty = true;
     if (p.method == ST_METHOD_SORTED && !capturing) {
@@ -394,8 +407,10 @@ int st_ctx_create(int32_t device, st_ctx **out) {
         lds > 0)
         c->max_lds = (size_t)lds;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&c->ev[0]);
-    if (e == hipSuccess) e = hipEventCreate(&c->ev[1]);
+    for (int k = 0; k < kTimerRing && e == hipSuccess; ++k) {
+        e = hipEventCreate(&c->tev[k][0]);
+        if (e == hipSuccess) e = hipEventCreate(&c->tev[k][1]);
+    }
     if (e != hipSuccess) {
         st_ctx_destroy(c);
         return fail(ST_E_DEVICE, "context creation: %s", hipGetErrorString(e));
@@ -781,12 +796,35 @@ int st_last_kernel_ms(st_ctx *ctx, double *ms, int32_t *launches) {
     ST_TRY(check_ctx(ctx));
     if (!ms || !launches) return fail(ST_E_INVALID, "null argument");
     *ms = -1.0;
-    *launches = ctx->timed_launches;
-    if (ctx->timed_launches <= 0) return ST_OK;
-    ST_HIP(hipEventSynchronize(ctx->ev[1]));
+    *launches = 0;
+    if (ctx->tcount == 0) return ST_OK;
+    const int slot = (int)((ctx->tcount - 1) % kTimerRing);
+    *launches = ctx->tlaunches[slot];
+    if (*launches <= 0) return ST_OK;
+    ST_HIP(hipEventSynchronize(ctx->tev[slot][1]));
     float t = 0.f;
-    ST_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
-    *ms = (double)t / ctx->timed_launches;
+    ST_HIP(hipEventElapsedTime(&t, ctx->tev[slot][0], ctx->tev[slot][1]));
+    *ms = (double)t / *launches;
+    return ST_OK;
+}
+
+int st_diag_kernel_times(st_ctx *ctx, int32_t max, double *ms, int32_t *launches, int32_t *count) {
+    ST_TRY(check_ctx(ctx));
+    if (!ms || !launches || !count || max < 1) return fail(ST_E_INVALID, "null argument or max < 1");
+    const int64_t have = std::min<int64_t>(ctx->tcount, kTimerRing);
+    const int32_t n = (int32_t)std::min<int64_t>(have, max);
+    *count = n;
+    if (n == 0) return ST_OK;
+    ST_HIP(hipEventSynchronize(ctx->tev[(ctx->tcount - 1) % kTimerRing][1]));
+    for (int32_t i = 0; i < n; ++i) {  // oldest first
+        const int slot = (int)((ctx->tcount - n + i) % kTimerRing);
+        launches[i] = ctx->tlaunches[slot];
+        ms[i] = -1.0;
+        if (launches[i] <= 0) continue;
+        float t = 0.f;
+        ST_HIP(hipEventElapsedTime(&t, ctx->tev[slot][0], ctx->tev[slot][1]));
+        ms[i] = (double)t / launches[i];
+    }
     return ST_OK;
 }
 
@@ -804,16 +842,16 @@ int st_diag_copy_bandwidth(st_ctx *ctx, int64_t bytes, int32_t reps, double *gbs
     hipStream_t s = ctx->stream;
     e = st::launch_fill_u32((uint32_t *)a, 0x3f800000u, (size_t)bytes / 4, s);
     if (e == hipSuccess) e = st::launch_copy(a, b, (size_t)bytes, s);  // warm-up
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
+    const int slot = (int)(ctx->tcount % kTimerRing);
+    if (e == hipSuccess) e = timer_start(ctx, s);
     for (int r = 0; r < reps && e == hipSuccess; ++r) e = st::launch_copy(a, b, (size_t)bytes, s);
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], s);
-    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[1]);
+    if (e == hipSuccess) e = timer_stop(ctx, s, reps);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->tev[slot][1]);
     float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->tev[slot][0], ctx->tev[slot][1]);
     (void)hipFree(a);
     (void)hipFree(b);
     if (e != hipSuccess) return fail(ST_E_DEVICE, "copy: %s", hipGetErrorString(e));
-    ctx->timed_launches = 0;
     *gbs = 2.0 * (double)bytes * reps / ((double)ms * 1e-3) / 1e9;
     return ST_OK;
 }

@@ -138,7 +138,7 @@ int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
     ST_TRY(table_for(b, dt_sub));
     hipStream_t s = b->ctx->stream;
     const int per = b->p.substeps_per_pass > 0 ? b->p.substeps_per_pass : nsub;
-    if (!capturing) ST_HIP(hipEventRecord(b->ctx->ev[0], s));
+    if (!capturing) ST_HIP(timer_start(b->ctx, s));
     int launches = 0;
     const int32_t n = (int32_t)b->clouds.size();
     const int periodic = b->p.boundary == ST_PERIODIC;
@@ -162,10 +162,7 @@ int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
         }
         ++launches;  // passes: the kernel time per pass spans both launches
     }
-    if (!capturing) {
-        ST_HIP(hipEventRecord(b->ctx->ev[1], s));
-        b->ctx->timed_launches = launches;
-    }
+    if (!capturing) ST_HIP(timer_stop(b->ctx, s, launches));
     ST_HIP(st::launch_reduce_stats_many(b->targets, (int32_t)b->clouds.size(), s));
     if (b->p.flush == ST_FLUSH_OWNER)
         for (st_field *f : b->fields) f->stray_dirty = true;

@@ -33,11 +33,17 @@ constexpr int kNF = 8;  // d, x, y, z, u, v, w, rho
 }  // namespace host
 }  // namespace st
 
+// Event pairs around the dominant kernel launches of the last kTimerRing timed calls
+// (st_evolve, batch and adjoint passes), so that timings of consecutive calls can be
+// read after one synchronisation.
+constexpr int kTimerRing = 64;
+
 struct st_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    int timed_launches = 0;  // launches between ev[0] and ev[1] of the last st_evolve
+    hipEvent_t tev[kTimerRing][2] = {};
+    int32_t tlaunches[kTimerRing] = {};  // launches between the pair of each slot
+    int64_t tcount = 0;                  // timed calls so far (slot = index % kTimerRing)
     size_t max_lds = 65536;  // LDS bytes one workgroup may use on this device
     // Clouds, fields, graphs, batches and adjoints hold a reference: st_ctx_destroy only
     // marks the context closed while any of them is alive; the last release frees it.
@@ -133,6 +139,10 @@ inline double substep_dt(int prec, double dt_fluid, int nsub) {
 template <class T>
 Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd);  // st_api.cpp
 int merge_source(st_field *f);
+// Timed region of one call on the context stream: start() records the start event of the
+// next ring slot, stop() its end event and launch count and makes it the newest slot.
+hipError_t timer_start(st_ctx *ctx, hipStream_t s);
+hipError_t timer_stop(st_ctx *ctx, hipStream_t s, int launches);
 // ST_E_INVALID unless the brick's halo tile fits in one workgroup's LDS.
 int check_tile(const st_ctx *ctx, int prec, const int32_t brick[3]);
 // Statistics slots for nslots workgroups (grown on demand).

@@ -104,6 +104,8 @@ SIGNATURES = {
     "st_graph_destroy": (C.c_int, [_vp]),
     "st_last_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     "st_diag_copy_bandwidth": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "st_diag_kernel_times": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32)]),
     "st_batch_create": (C.c_int, [_vp, C.c_int32, _pp, _pp, C.POINTER(st_params), C.c_int32, C.c_int32, _pp]),
     "st_batch_destroy": (C.c_int, [_vp]),
     "st_batch_sort": (C.c_int, [_vp]),

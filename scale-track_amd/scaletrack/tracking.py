@@ -53,6 +53,15 @@ class Context:
         check(lib.st_last_kernel_ms(self.handle, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def kernel_times(self, max_calls=64):
+        """Per-launch kernel ms and launch counts of the last timed calls, oldest first
+        (one synchronisation for all of them)."""
+        ms = (C.c_double * max_calls)()
+        nl = (C.c_int32 * max_calls)()
+        cnt = C.c_int32()
+        check(lib.st_diag_kernel_times(self.handle, int(max_calls), ms, nl, C.byref(cnt)))
+        return [(ms[i], nl[i]) for i in range(cnt.value)]
+
     def copy_bandwidth(self, nbytes=1 << 30, reps=10):
         """Measured device-to-device streaming copy, read + write GB/s."""
         g = C.c_double()

@@ -141,3 +141,27 @@ def test_hand_cases_gpu(ctx, case, precision, method):
     for c, vals in ex["src_over_m"].items():
         exp[int(c)] = np.asarray(vals) * mass
     np.testing.assert_allclose(src, exp, rtol=1e-6 if precision == stk.FP32 else 1e-15, atol=0)
+
+
+def test_kernel_times_ring(ctx):
+    """st_diag_kernel_times: one entry per timed call, oldest first, the newest equal to
+    st_last_kernel_ms; the ring keeps the last 64 calls."""
+    case = inputs.taylor_green_box(np_parcels=20000, ncell=16, boundary=1)
+    mesh = stk.Mesh(case.n, case.origin, case.ending)
+    field = stk.TwoWayEulerian(ctx, mesh, stk.FP64)
+    field.upload_U(case.U)
+    chunk = stk.Chunk(ctx, case.np, case.mu_c, case.rho_d, case.rho_d_by_rho_c, stk.FP64)
+    chunk.set_state(*case.state())
+    chunk.upload()
+    p = stk.params(boundary=stk.PERIODIC, substeps_per_pass=1)
+    for k in range(70):
+        stk.evolve_device(ctx, chunk, field, case.dt_fluid, 1 + k % 3, p, want_bbox=False)
+    times = ctx.kernel_times(100)
+    assert len(times) == 64
+    assert [n for _, n in times] == [1 + k % 3 for k in range(6, 70)]
+    assert all(ms > 0 for ms, _ in times)
+    last = ctx.last_kernel_ms()
+    assert last == times[-1]
+    assert len(ctx.kernel_times(5)) == 5 and ctx.kernel_times(5)[-1] == times[-1]
+    chunk.close()
+    field.close()
