@@ -45,10 +45,14 @@ __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
     double m, f, den;
     parcel_consts<double>(a.ph, a.rho != nullptr, dia, rho, m, f, den);
     const double dt = a.ph.dt;
+    // The reverse sweep multiplies by 1 / (1 + f) instead of dividing (its results are
+    // compared with the derivative to rounding, not bit for bit); the replayed forward
+    // keeps the forward's exact divisions, so it takes the forward's decisions.
+    const double rden = 1.0 / den;
 
     double xb0 = a.xb[p], xb1 = a.xb[n + p], xb2 = a.xb[2 * n + p];
     double vb0 = a.vb[p], vb1 = a.vb[n + p], vb2 = a.vb[2 * n + p];
-    double fb = 0.0, mb = 0.0;
+    double fb = 0.0, mb = 0.0;  // fb without its factor 1 / (1 + f), applied at the end
 
     const double cx = a.ck[p], cy = a.ck[n + p], cz = a.ck[2 * n + p];
     const double cu = a.ck[3 * n + p], cv = a.ck[4 * n + p], cw = a.ck[5 * n + p];
@@ -130,8 +134,8 @@ __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
                     } else {
                         const double ga = gg[ax];
                         const double w = *vbp[ax] + xbv[ax] * dt - m * ga;
-                        *vbp[ax] = w / den + m * ga;
-                        fb += w * (uu[ax] - nv[ax]) / den;
+                        *vbp[ax] = w * rden + m * ga;
+                        fb += w * (uu[ax] - nv[ax]);
                         mb += ga * (pv[ax] - nv[ax]);
                     }
                 }
@@ -144,7 +148,7 @@ __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
     a.vb[p] = vb0;
     a.vb[n + p] = vb1;
     a.vb[2 * n + p] = vb2;
-    a.fbar[p] += fb;
+    a.fbar[p] += fb * rden;
     a.mbar[p] += mb;
 }
 

@@ -215,6 +215,11 @@ class Chunk:
         check(lib.st_cloud_download(self.handle, *[_ptr(a) for a in arrs]))
         return arrs
 
+    def sort(self, ctx, eulerian, p=None):
+        """Sort the device parcels by brick of `eulerian`'s mesh now (st_cloud_sort);
+        otherwise the first sorted step after an upload sorts."""
+        check(lib.st_cloud_sort(ctx.handle, self.handle, eulerian.handle, C.byref(p) if p is not None else None))
+
     def stray_fraction(self):
         f = C.c_double()
         check(lib.st_cloud_stray_fraction(self.handle, C.byref(f)))
