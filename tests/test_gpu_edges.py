@@ -1,0 +1,110 @@
+"""Edge cases of the GPU paths: empty and one-parcel clouds, a non-cubic mesh that no
+brick divides, and the size and range limits the library turns into errors."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import scaletrack as stk
+from scaletrack import _lib, inputs
+from oracle import st_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def cloud(ctx, case, precision):
+    dt = np.float64 if precision == stk.FP64 else np.float32
+    field = stk.TwoWayEulerian(ctx, stk.Mesh(case.n, case.origin, case.ending), precision)
+    field.upload_U(case.U.astype(dt))
+    field.zero_source()
+    chunk = stk.Chunk(ctx, case.np, case.mu_c, case.rho_d, case.rho_d_by_rho_c, precision)
+    chunk.set_state(*[a.astype(dt) for a in case.state()])
+    chunk.upload()
+    return chunk, field
+
+
+def odd_case(np_parcels, dtype, boundary, seed=31):
+    n = (37, 23, 5)
+    o, e = (0.0, -0.1, 0.0), (1.0, 0.6, 0.2)
+    U = inputs.taylor_green(n, o, e, phase=(0.4, 1.3, 2.0), dtype=dtype)
+    x, y, z = inputs.positions(seed, np_parcels, o, e, dtype=dtype)
+    d = inputs.lognormal_diameters(seed, np_parcels, dtype=dtype)
+    zero = np.zeros(np_parcels, dtype=dtype)
+    return inputs.Case(n, o, e, U, d, x, y, z, zero, zero.copy(), zero.copy(), boundary=boundary)
+
+
+@pytest.mark.parametrize("method", [stk.PLAIN, stk.SORTED])
+@pytest.mark.parametrize("precision", [stk.FP64, stk.FP32])
+def test_empty_cloud(ctx, method, precision):
+    case = odd_case(0, np.float64 if precision == stk.FP64 else np.float32, 0)
+    chunk, field = cloud(ctx, case, precision)
+    bb = stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, stk.params(method=method))
+    assert np.all(field.download_source() == 0.0)
+    assert np.all(bb[0] == np.inf) and np.all(bb[1] == -np.inf)
+    chunk.close()
+    field.close()
+
+
+@pytest.mark.parametrize("method", [stk.PLAIN, stk.SORTED])
+@pytest.mark.parametrize("precision", [stk.FP64, stk.FP32])
+@pytest.mark.parametrize("boundary", [stk.WALL, stk.PERIODIC])
+@pytest.mark.parametrize("np_parcels", [1, 5000])
+def test_non_cubic_mesh_partial_bricks(ctx, method, precision, boundary, np_parcels):
+    """37x23x5 cells with the default 32x8x2 brick: partial bricks on every axis, and
+    with one parcel all bricks but one are empty."""
+    dt = np.float64 if precision == stk.FP64 else np.float32
+    case = odd_case(np_parcels, dt, boundary)
+    chunk, field = cloud(ctx, case, precision)
+    p = stk.params(boundary=boundary, method=method, substeps_per_pass=1)
+    state = case.state()
+    for step in range(3):
+        field.zero_source()
+        bb = stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p)
+        ref_src, ref_bb, state = st_oracle.evolve_case(
+            case, state=state, precision=st_oracle.FP64 if precision == stk.FP64 else st_oracle.FP32,
+            substeps_per_pass=1)
+        src = field.download_source()
+        scale = max(np.abs(ref_src).max(), 1e-300)
+        assert np.abs(src - ref_src).max() <= 1e-12 * scale, step
+        np.testing.assert_array_equal(np.concatenate(bb), ref_bb)
+    for name, a, b in zip("dxyzuvw", chunk.download(), state):
+        assert np.array_equal(a, b), name
+    chunk.close()
+    field.close()
+
+
+def test_cloud_size_limit(ctx):
+    """Parcel arrays are addressed with 32-bit byte offsets: a cloud whose arrays reach
+    4 GiB is refused before any allocation."""
+    h = C.c_void_p()
+    for prec, n in ((stk.FP64, 1 << 29), (stk.FP32, 1 << 30)):
+        r = _lib.lib.st_cloud_create(ctx.handle, prec, n, 1e-3, 1000.0, 1.0, 1000.0, 0, C.byref(h))
+        assert r == _lib.ST_E_RANGE
+        assert b"exceed 4 GiB" in _lib.lib.st_last_error()
+    r = _lib.lib.st_cloud_create(ctx.handle, stk.FP64, -1, 1e-3, 1000.0, 1.0, 1000.0, 0, C.byref(h))
+    assert r == _lib.ST_E_RANGE
+
+
+def test_brick_limits(ctx):
+    """A brick above 2048 cells, or whose tile with its halo needs more LDS than one
+    workgroup may use, is refused with the reason."""
+    case = odd_case(100, np.float64, 0)
+    chunk, field = cloud(ctx, case, stk.FP64)
+    with pytest.raises(_lib.ScaleTrackError, match="at most 2048"):
+        stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, stk.params(brick=(64, 64, 64)))
+    with pytest.raises(_lib.ScaleTrackError, match="LDS"):
+        stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, stk.params(brick=(64, 32, 1)))
+    chunk.close()
+    field.close()
+
+
+def test_wall_parcel_outside_mesh_is_refused_by_the_sort(ctx):
+    """Wall mode keeps parcels inside the box; a parcel uploaded outside it has no brick,
+    and the sorted method reports it instead of misplacing it."""
+    case = odd_case(100, np.float64, 0)
+    case.x[7] = 1.5
+    chunk, field = cloud(ctx, case, stk.FP64)
+    with pytest.raises(_lib.ScaleTrackError, match="outside the mesh"):
+        stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, stk.params(method=stk.SORTED))
+    chunk.close()
+    field.close()
