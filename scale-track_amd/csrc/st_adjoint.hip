@@ -19,6 +19,8 @@
 // registers (nested recompute: segment blocks of B are replayed last block first).
 #include "st_device.h"
 
+#include <algorithm>
+
 namespace st {
 
 struct AdjArgs {
@@ -26,6 +28,7 @@ struct AdjArgs {
     const double *ck;       // checkpoint of this segment: x, y, z, u, v, w (SoA, n each)
     const double *U;        // cell velocity, AoS
     const double *G;        // dL/dS, AoS
+    const double *UG;       // per cell U xyz then G xyz (48 B, 16-byte aligned)
     double *xb;             // adjoint of x, y, z (SoA, 3n)
     double *vb;             // adjoint of u, v, w (SoA, 3n)
     double *fbar, *mbar;    // accumulated adjoints of f and m per parcel
@@ -34,6 +37,13 @@ struct AdjArgs {
     int32_t K;              // substeps in this segment
     uint32_t np;
 };
+
+__device__ __forceinline__ void load_ug(const double *ug, uint32_t c, double2 &r0, double2 &r1, double2 &r2) {
+    const double2 *q = (const double2 *)(ug + 6u * c);
+    r0 = q[0];
+    r1 = q[1];
+    r2 = q[2];
+}
 
 template <bool PERIODIC, int B>
 __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
@@ -114,14 +124,17 @@ __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
                 ++t;
             }
         }
-        // reverse this block
+        // reverse this block; the U and G of each substep's cell (one 48-byte record)
+        // are requested one substep ahead, so the gather overlaps a substep of work
         double nvx = vx, nvy = vy, nvz = vz;  // velocity after the substep being reversed
+        double2 r0, r1, r2;
 #pragma unroll
         for (int j = B - 1; j >= 0; --j) {
             if (j < nb) {
-                const uint32_t c = rc[j];
-                const double *uu = a.U + 3u * c;
-                const double *gg = a.G + 3u * c;
+                if (j == nb - 1) load_ug(a.UG, rc[j], r0, r1, r2);
+                const double uu[3] = {r0.x, r0.y, r1.x};
+                const double gg[3] = {r1.y, r2.x, r2.y};
+                if (j > 0) load_ug(a.UG, rc[j - 1], r0, r1, r2);
                 const unsigned bj = (rb >> (3 * j)) & 7u;
                 const double pv[3] = {rv[j][0], rv[j][1], rv[j][2]};
                 const double nv[3] = {nvx, nvy, nvz};
@@ -236,6 +249,25 @@ __global__ void k_scatter_pid(const double *src, const int32_t *pid, double *dst
     dst[pid[p]] = src[p];
 }
 
+__global__ void k_interleave3(const double *a, const double *b, double *dst, size_t n) {
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        double *q = dst + 6 * c;
+        q[0] = a[3 * c];
+        q[1] = a[3 * c + 1];
+        q[2] = a[3 * c + 2];
+        q[3] = b[3 * c];
+        q[4] = b[3 * c + 1];
+        q[5] = b[3 * c + 2];
+    }
+}
+
+hipError_t launch_interleave3(const double *a, const double *b, double *dst, size_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    k_interleave3<<<(unsigned)blocks, 256, 0, s>>>(a, b, dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjoint_segment(const AdjArgsHost &h, hipStream_t s) {
     AdjArgs a;
     a.d = h.d;
@@ -243,6 +275,7 @@ hipError_t launch_adjoint_segment(const AdjArgsHost &h, hipStream_t s) {
     a.ck = h.ck;
     a.U = h.U;
     a.G = h.G;
+    a.UG = h.UG;
     a.xb = h.xb;
     a.vb = h.vb;
     a.fbar = h.fbar;

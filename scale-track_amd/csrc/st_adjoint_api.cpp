@@ -30,6 +30,7 @@ struct st_adjoint {
     double *fbar = nullptr;  // n
     double *mbar = nullptr;  // n
     double *G = nullptr;     // 3 * ncell
+    double *UG = nullptr;    // 6 * ncell: U and G of each cell side by side (reverse sweep)
     double *tmp = nullptr;   // 6n staging
     double *scal = nullptr;  // [0] loss, [1] d L / d cd
 };
@@ -39,7 +40,7 @@ namespace {
 int destroy(st_adjoint *a) {
     if (!a) return ST_OK;
     quiesce_ctx(a->ctx);
-    for (double *q : {a->ck, a->xb, a->vb, a->vseed, a->fbar, a->mbar, a->G, a->tmp, a->scal})
+    for (double *q : {a->ck, a->xb, a->vb, a->vseed, a->fbar, a->mbar, a->G, a->UG, a->tmp, a->scal})
         if (q) (void)hipFree(q);
     st_ctx *ctx = a->ctx;
     delete a;
@@ -86,6 +87,7 @@ int st_adjoint_create(st_ctx *ctx, st_cloud *c, st_field *f, int32_t nsub, int32
     if (e == hipSuccess) e = hipMalloc((void **)&a->fbar, n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->mbar, n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->G, nc3 * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->UG, 2 * nc3 * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->tmp, 6 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->scal, 2 * 8);
     if (e == hipSuccess) e = st::launch_fill_f64(a->G, 0.0, nc3, ctx->stream);
@@ -190,6 +192,8 @@ int st_adjoint_backward(st_adjoint *a) {
     h.rho = cur.rho;
     h.U = (const double *)f->U;
     h.G = a->G;
+    h.UG = a->UG;
+    ST_HIP(st::launch_interleave3(h.U, a->G, a->UG, (size_t)f->ncell, s));
     h.xb = a->xb;
     h.vb = a->vb;
     h.fbar = a->fbar;

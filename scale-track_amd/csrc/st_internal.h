@@ -132,7 +132,7 @@ hipError_t launch_copy(const void *a, void *b, size_t bytes, hipStream_t s);
 
 // Adjoint (st_adjoint.hip), FP64.
 struct AdjArgsHost {
-    const double *d, *rho, *ck, *U, *G;
+    const double *d, *rho, *ck, *U, *G, *UG;
     double *xb, *vb, *fbar, *mbar;
     Geo g;
     Phys<double> ph;
@@ -141,6 +141,8 @@ struct AdjArgsHost {
     int periodic;
 };
 hipError_t launch_adjoint_segment(const AdjArgsHost &h, hipStream_t s);
+// dst[6c .. 6c+5] = a[3c .. 3c+2], b[3c .. 3c+2] for c < n.
+hipError_t launch_interleave3(const double *a, const double *b, double *dst, size_t n, hipStream_t s);
 hipError_t launch_adjoint_grads(const double *d, const double *rho, const double *fbar, const double *mbar,
                                 double *gd, double *grho, double *gcd, const Phys<double> &ph, uint32_t np,
                                 hipStream_t s);
