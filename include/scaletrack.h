@@ -161,8 +161,11 @@ int st_last_kernel_ms(st_ctx *ctx, double *ms, int32_t *launches);
  * (asyncParallelTracking.jl:1098-1125); a batch replaces those launches.  All clouds
  * share the precision and params (boundary, flush, block, substeps_per_pass, cd).
  * target_parcels_per_brick > 0 picks each cloud's brick so that a brick holds about
- * that many parcels (0: params->brick for all).  blocks_per_cu caps the persistent grid
- * per CU (0: as many as fit).  Clouds are sorted at creation; st_batch_sort re-sorts. */
+ * that many parcels (0: params->brick for all).  blocks_per_cu > 0 caps the persistent
+ * grid at that many workgroups per CU and, with target_parcels_per_brick, also bounds
+ * every cloud's brick so that its tile fits that share of a CU's LDS (0: as many as
+ * fit, bricks bounded by one workgroup's LDS only).  Clouds are sorted at creation;
+ * st_batch_sort re-sorts. */
 typedef struct st_batch st_batch;
 int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *const *fields,
                     const st_params *params, int32_t target_parcels_per_brick,
@@ -182,7 +185,7 @@ int st_batch_info(const st_batch *batch, int32_t *total_items, int32_t *grid,
 
 /* Extension: autotuner of a batch.  Times every candidate (parcels per brick in
  * {1024 .. 16384}, block in {256, 512, 1024}, persistent workgroups per CU capped at
- * {all, 4, 2}) over `trials` passes of one substep (or base->substeps_per_pass) and
+ * {all, 4, 3, 2}) over `trials` passes of one substep (or base->substeps_per_pass) and
  * restores the clouds and sources afterwards.  The result is cached by device and
  * workload in cache_path (JSON lines; NULL or "" disables the cache). */
 typedef struct st_tune_result {

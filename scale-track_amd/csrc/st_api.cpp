@@ -406,6 +406,10 @@ int st_ctx_create(int32_t device, st_ctx **out) {
     if (e == hipSuccess && hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
         lds > 0)
         c->max_lds = (size_t)lds;
+    if (e == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess &&
+        lds > 0)
+        c->lds_per_cu = (size_t)lds;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int k = 0; k < kTimerRing && e == hipSuccess; ++k) {
         e = hipEventCreate(&c->tev[k][0]);

@@ -39,6 +39,10 @@ struct st_batch {
 
 namespace {
 
+// Static LDS of the batch kernel (its statistics reduction and item slot), kept free
+// next to the tile when bricks are sized for a number of workgroups per CU.
+constexpr size_t kBatchStaticLds = 1024;
+
 // Brick of about `cells` cells: powers of two, x extent largest (rows stay contiguous),
 // at most `max_cells` including the one-cell halo.
 void auto_brick(int64_t cells, const int32_t n[3], int64_t max_cells, int32_t b[3]) {
@@ -239,8 +243,20 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
         if (target_parcels_per_brick > 0) {
             const double want = target_parcels_per_brick / std::max(density[i], 1e-9);
             const size_t per_cell = 3 * (sizeof(double) + (clouds[i]->prec == ST_FP64 ? 8 : 4));
-            auto_brick((int64_t)std::max(1.0, want), fields[i]->mesh.n, (int64_t)(ctx->max_lds / per_cell), pc.brick);
-            while (check_tile(ctx, clouds[i]->prec, pc.brick) != ST_OK && pc.brick[0] * pc.brick[1] * pc.brick[2] > 1) {
+            // With a cap of k workgroups per CU, every cloud's tile fits a k-th of the
+            // CU's LDS, so k workgroups can be resident whichever clouds they hold
+            // (one LDS size serves the whole launch: the largest tile's).
+            size_t budget = ctx->max_lds;
+            if (blocks_per_cu > 0) {
+                const size_t share = ctx->lds_per_cu / (size_t)blocks_per_cu;
+                budget = std::min(budget, share > kBatchStaticLds ? share - kBatchStaticLds : 0);
+            }
+            auto_brick((int64_t)std::max(1.0, want), fields[i]->mesh.n, (int64_t)(budget / per_cell), pc.brick);
+            auto tile = [&] {
+                return clouds[i]->prec == ST_FP64 ? st::tile_lds_bytes<double>(pc.brick) : st::tile_lds_bytes<float>(pc.brick);
+            };
+            while ((tile() > budget || check_tile(ctx, clouds[i]->prec, pc.brick) != ST_OK) &&
+                   pc.brick[0] * pc.brick[1] * pc.brick[2] > 1) {
                 int ax = pc.brick[0] >= pc.brick[1] && pc.brick[0] >= pc.brick[2] ? 0 : (pc.brick[1] >= pc.brick[2] ? 1 : 2);
                 pc.brick[ax] = std::max(1, pc.brick[ax] / 2);
             }

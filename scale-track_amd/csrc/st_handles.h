@@ -44,7 +44,8 @@ struct st_ctx {
     hipEvent_t tev[kTimerRing][2] = {};
     int32_t tlaunches[kTimerRing] = {};  // launches between the pair of each slot
     int64_t tcount = 0;                  // timed calls so far (slot = index % kTimerRing)
-    size_t max_lds = 65536;  // LDS bytes one workgroup may use on this device
+    size_t max_lds = 65536;      // LDS bytes one workgroup may use on this device
+    size_t lds_per_cu = 163840;  // LDS bytes of one CU (all resident workgroups together)
     // Clouds, fields, graphs, batches and adjoints hold a reference: st_ctx_destroy only
     // marks the context closed while any of them is alive; the last release frees it.
     int refs = 0;

@@ -538,18 +538,26 @@ __global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
 // from a global counter until none are left.  item0[c] is the first item of cloud c
 // (clouds ordered densest first, so the longest items are handed out first); every
 // workgroup leaves once the counter passes item0[ncloud].
+//
+// A workgroup claims its next item as it starts the current one: the counter's
+// atomic (a round trip to the memory side, since the counter is shared by all XCDs)
+// then completes under the brick's tile staging instead of stalling the workgroup
+// between items.  The FP32 instances are held to 80 VGPRs (6 waves per SIMD), the
+// budget of the single-cloud kernel: the loop over items otherwise keeps enough
+// per-thread values live across items to drop the occupancy by a wave.
 template <class T, bool PERIODIC, bool RHO>
-__global__ void __launch_bounds__(1024) k_pass_batch(const PassArgs<T> *__restrict__ descs,
-                                                     const int32_t *__restrict__ item0, int32_t ncloud,
-                                                     int32_t nsteps, int stats, unsigned int *counter) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4)))
+k_pass_batch(const PassArgs<T> *__restrict__ descs, const int32_t *__restrict__ item0, int32_t ncloud,
+             int32_t nsteps, int stats, unsigned int *counter) {
     extern __shared__ double lds[];
     __shared__ int32_t s_item;
     const int32_t total = item0[ncloud];
-    for (;;) {
-        if (threadIdx.x == 0) s_item = (int32_t)atomicAdd(counter, 1u);
-        __syncthreads();
-        const int32_t item = __builtin_amdgcn_readfirstlane(s_item);
-        if (item >= total) break;
+    if (threadIdx.x == 0) s_item = (int32_t)atomicAdd(counter, 1u);
+    __syncthreads();
+    int32_t item = __builtin_amdgcn_readfirstlane(s_item);
+    while (item < total) {
+        int32_t next = 0;
+        if (threadIdx.x == 0) next = (int32_t)atomicAdd(counter, 1u);
         int32_t lo = 0, hi = ncloud - 1;  // last cloud with item0 <= item
         while (lo < hi) {
             const int32_t mid = (lo + hi + 1) >> 1;
@@ -561,7 +569,10 @@ __global__ void __launch_bounds__(1024) k_pass_batch(const PassArgs<T> *__restri
         const int32_t c = __builtin_amdgcn_readfirstlane(lo);
         const PassArgs<T> a = descs[c];
         process_brick<T, PERIODIC, RHO>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
-        __syncthreads();  // s_item is rewritten only after every thread has read it
+        __syncthreads();  // the flush's tile reads are done before the next staging
+        if (threadIdx.x == 0) s_item = next;
+        __syncthreads();
+        item = __builtin_amdgcn_readfirstlane(s_item);
     }
 }
 

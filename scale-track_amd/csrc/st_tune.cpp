@@ -152,7 +152,7 @@ int st_autotune_batch(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field 
     }
     const int targets[] = {1024, 2048, 4096, 8192, 16384};
     const int blocks[] = {256, 512, 1024};
-    const int caps[] = {0, 4, 2};
+    const int caps[] = {0, 4, 3, 2};
     st_tune_result best{};
     best.ms_per_pass = INFINITY;
     const int nsub = p.substeps_per_pass > 0 ? p.substeps_per_pass : 1;

@@ -297,7 +297,9 @@ class Graph:
 class Batch:
     """Independent clouds advanced together by one persistent launch per pass (st_batch).
     `target_parcels_per_brick` > 0 sizes each cloud's brick to hold about that many
-    parcels; `blocks_per_cu` caps the persistent grid (0: as many as fit)."""
+    parcels; `blocks_per_cu` > 0 caps the persistent grid at that many workgroups per CU
+    and bounds each cloud's brick so its tile fits that share of a CU's LDS (0: as many
+    as fit)."""
 
     def __init__(self, ctx, chunks, fields, p=None, target_parcels_per_brick=4096, blocks_per_cu=0):
         n = len(chunks)
