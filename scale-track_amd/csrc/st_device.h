@@ -37,8 +37,9 @@ __device__ __forceinline__ void fold(int32_t I, int32_t N, int32_t &f, int32_t &
     }
 }
 
+// i clamped to [0, n) (n >= 1): the median of i, 0 and n - 1, one v_med3_i32.
 __device__ __forceinline__ int32_t clampi(int32_t i, int32_t n) {
-    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    return min(max(i, 0), n - 1);
 }
 
 struct Cell {
@@ -56,15 +57,23 @@ __device__ __forceinline__ uint32_t lin(const Geo &g, const Cell &cl) {
 // Same folded cell: the linear index is one-to-one on folded indices, so comparing the
 // three axes decides what comparing Cell::c does.
 __device__ __forceinline__ bool same_cell(const Cell &a, const Cell &b) {
-    return a.f[0] == b.f[0] && a.f[1] == b.f[1] && a.f[2] == b.f[2];
+    return (a.f[0] == b.f[0]) & (a.f[1] == b.f[1]) & (a.f[2] == b.f[2]);  // one mask, no branches
 }
 
 __device__ __forceinline__ void cell_wall(const Geo &g, const int32_t I[3], Cell &cl) {
-    int32_t p0, p1, p2;
-    fold(I[0], g.n[0], cl.f[0], p0);
-    fold(I[1], g.n[1], cl.f[1], p1);
-    fold(I[2], g.n[2], cl.f[2], p2);
-    cl.part = p0 + p1 + p2;
+    if (((uint32_t)I[0] < (uint32_t)g.n[0]) & ((uint32_t)I[1] < (uint32_t)g.n[1]) &
+        ((uint32_t)I[2] < (uint32_t)g.n[2])) {  // inside the mesh: one branch for the three axes
+        cl.f[0] = I[0];
+        cl.f[1] = I[1];
+        cl.f[2] = I[2];
+        cl.part = 0;
+    } else {
+        int32_t p0, p1, p2;
+        fold(I[0], g.n[0], cl.f[0], p0);
+        fold(I[1], g.n[1], cl.f[1], p1);
+        fold(I[2], g.n[2], cl.f[2], p2);
+        cl.part = p0 + p1 + p2;
+    }
     cl.c = lin(g, cl);
 }
 
@@ -137,7 +146,12 @@ template <>
 struct Quot<double> {
     double den;
     __device__ __forceinline__ explicit Quot(double d) : den(d) {}
-    __device__ __forceinline__ double operator()(double a) const { return a / den; }
+    __device__ __forceinline__ void three(double ax, double ay, double az, double &rx, double &ry,
+                                          double &rz) const {
+        rx = ax / den;
+        ry = ay / den;
+        rz = az / den;
+    }
 };
 // For floats, a / den == (float)((double)a * RN64(1 / den)) whenever the quotient is
 // zero or outside the FP32 subnormal range.  The double product is within 2^-52
@@ -153,10 +167,21 @@ struct Quot<float> {
     float den;
     double r;
     __device__ __forceinline__ explicit Quot(float d) : den(d), r(1.0 / (double)d) {}
-    __device__ __forceinline__ float operator()(float a) const {
-        const double q = (double)a * r;
-        if (q != 0.0 && fabs(q) < 0x1p-126) return a / den;
-        return (float)q;
+    // The three components of a substep with one test: when the smallest rounded
+    // product is at least 2^-126 every product is correct (a product that rounds up to
+    // 2^-126 lies in [2^-127, 2^-126), where a quotient can sit on a midpoint only if
+    // den is a power of two, and then the product is exact).  Otherwise each component
+    // whose rounded product is below 2^-126 and whose numerator is nonzero takes the
+    // float division (a zero numerator gives the same signed zero either way).
+    __device__ __forceinline__ void three(float ax, float ay, float az, float &rx, float &ry, float &rz) const {
+        rx = (float)((double)ax * r);
+        ry = (float)((double)ay * r);
+        rz = (float)((double)az * r);
+        if (fminf(fminf(fabsf(rx), fabsf(ry)), fabsf(rz)) < 0x1p-126f) {
+            if (fabsf(rx) < 0x1p-126f && ax != 0.0f) rx = ax / den;
+            if (fabsf(ry) < 0x1p-126f && ay != 0.0f) ry = ay / den;
+            if (fabsf(rz) < 0x1p-126f && az != 0.0f) rz = az / den;
+        }
     }
 };
 
@@ -171,9 +196,7 @@ __device__ __forceinline__ unsigned advance(const Geo &g, const Phys<T> &ph, T f
                                             int32_t part0, T &nx, T &ny, T &nz, T &nvx, T &nvy, T &nvz,
                                             Cell &nc) {
     // :1295-1296
-    nvx = den(vx + f * ux);
-    nvy = den(vy + f * uy);
-    nvz = den(vz + f * uz);
+    den.three(vx + f * ux, vy + f * uy, vz + f * uz, nvx, nvy, nvz);
     nx = px + nvx * ph.dt;
     ny = py + nvy * ph.dt;
     nz = pz + nvz * ph.dt;
@@ -200,32 +223,45 @@ __device__ __forceinline__ unsigned advance(const Geo &g, const Phys<T> &ph, T f
             if (bounced) locate<T, false>(g, nx, ny, nz, I, nc);  // :1333-1335
         }
     } else {
-        // periodic extension: wrap each axis that left [0, N) by one period
+        // periodic extension: wrap each axis that left [0, N) by one period (one
+        // unsigned test per axis and one branch for the three on the common path)
         I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
         I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
         I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
-        if (I[0] < 0) {
-            nx = nx + ph.L[0];
-            I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
-        } else if (I[0] >= g.n[0]) {
-            nx = nx - ph.L[0];
-            I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
+        if (((uint32_t)I[0] >= (uint32_t)g.n[0]) | ((uint32_t)I[1] >= (uint32_t)g.n[1]) |
+            ((uint32_t)I[2] >= (uint32_t)g.n[2])) {
+            if (I[0] < 0) {
+                nx = nx + ph.L[0];
+                I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
+            } else if (I[0] >= g.n[0]) {
+                nx = nx - ph.L[0];
+                I[0] = axis_floor((double)nx, g.o[0], g.rd[0]);
+            }
+            if (I[1] < 0) {
+                ny = ny + ph.L[1];
+                I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
+            } else if (I[1] >= g.n[1]) {
+                ny = ny - ph.L[1];
+                I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
+            }
+            if (I[2] < 0) {
+                nz = nz + ph.L[2];
+                I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
+            } else if (I[2] >= g.n[2]) {
+                nz = nz - ph.L[2];
+                I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
+            }
+            // a wrapped coordinate that rounds onto the far face is clamped, as a
+            // fresh location would be
+            I[0] = clampi(I[0], g.n[0]);
+            I[1] = clampi(I[1], g.n[1]);
+            I[2] = clampi(I[2], g.n[2]);
         }
-        if (I[1] < 0) {
-            ny = ny + ph.L[1];
-            I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
-        } else if (I[1] >= g.n[1]) {
-            ny = ny - ph.L[1];
-            I[1] = axis_floor((double)ny, g.o[1], g.rd[1]);
-        }
-        if (I[2] < 0) {
-            nz = nz + ph.L[2];
-            I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
-        } else if (I[2] >= g.n[2]) {
-            nz = nz - ph.L[2];
-            I[2] = axis_floor((double)nz, g.o[2], g.rd[2]);
-        }
-        cell_clamped(g, I, nc);
+        nc.f[0] = I[0];  // in [0, N) on both paths: the clamp of cell_clamped is a no-op
+        nc.f[1] = I[1];
+        nc.f[2] = I[2];
+        nc.part = 0;
+        nc.c = lin(g, nc);
     }
     return bounced;
 }

@@ -19,18 +19,24 @@
 namespace st {
 
 // Everything in HBM: the plain method.
+//
+// An environment hands out a Slot per residence (where the cell's velocity and sums
+// live), so a parcel's deposit into the cell it leaves reuses what was worked out when
+// it entered that cell.
 template <class T>
 struct GlobalEnv {
     const T *__restrict__ U;
     double *__restrict__ src;
+    struct Slot {};
     __device__ __forceinline__ void enter(const Cell &) {}
-    __device__ __forceinline__ void load_u(const Cell &cl, T &ux, T &uy, T &uz) const {
+    __device__ __forceinline__ Slot slot(const Cell &) const { return Slot{}; }
+    __device__ __forceinline__ void load_u(const Cell &cl, const Slot &, T &ux, T &uy, T &uz) const {
         const T *q = U + 3u * cl.c;
         ux = q[0];
         uy = q[1];
         uz = q[2];
     }
-    __device__ __forceinline__ void deposit(const Cell &cl, double a, double b, double c) {
+    __device__ __forceinline__ void deposit(const Cell &cl, const Slot &, double a, double b, double c) {
         double *q = src + 3u * cl.c;
         atomic_add_global(q, a);
         atomic_add_global(q + 1, b);
@@ -76,9 +82,18 @@ struct TileEnv {
         const uint32_t c = (uint32_t)(cl.f[2] - h2 - 1);
         if (!(a < e0 && b < e1 && c < e2)) ++strays;
     }
-    __device__ __forceinline__ void load_u(const Cell &cl, T &ux, T &uy, T &uz) const {
-        uint32_t l;
-        if (local(cl, l)) {
+    struct Slot {
+        uint32_t l;  // tile index (meaningful when in)
+        bool in;     // the cell lies in the tile
+    };
+    __device__ __forceinline__ Slot slot(const Cell &cl) const {
+        Slot sl;
+        sl.in = local(cl, sl.l);
+        return sl;
+    }
+    __device__ __forceinline__ void load_u(const Cell &cl, const Slot &sl, T &ux, T &uy, T &uz) const {
+        if (sl.in) {
+            const uint32_t l = sl.l;
             ux = Ut[3u * l];
             uy = Ut[3u * l + 1];
             uz = Ut[3u * l + 2];
@@ -90,10 +105,9 @@ struct TileEnv {
             __builtin_amdgcn_s_waitcnt(0);
         }
     }
-    __device__ __forceinline__ void deposit(const Cell &cl, double a, double b, double c) {
-        uint32_t l;
-        if (local(cl, l)) {
-            ST_LDS double *q = St + 3u * l;
+    __device__ __forceinline__ void deposit(const Cell &cl, const Slot &sl, double a, double b, double c) {
+        if (sl.in) {
+            ST_LDS double *q = St + 3u * sl.l;
             atomic_add_lds(q, a);
             atomic_add_lds(q + 1, b);
             atomic_add_lds(q + 2, c);
@@ -152,8 +166,9 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     locate<T, PERIODIC>(g, px, py, pz, I, cell);  // :1286
     const int32_t part0 = cell.part;
     env.enter(cell);
+    typename Env::Slot sl = env.slot(cell);
     T ux, uy, uz;
-    env.load_u(cell, ux, uy, uz);  // :1287-1288
+    env.load_u(cell, sl, ux, uy, uz);  // :1287-1288
     double s0x = (double)vx, s0y = (double)vy, s0z = (double)vz;  // :1289
     prefetch();
 
@@ -166,12 +181,13 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
         if (b & 2u) s0y = s0y - 2.0 * (double)vy;
         if (b & 4u) s0z = s0z - 2.0 * (double)vz;
         if (t != nsteps && !same_cell(nc, cell)) {  // :1342-1348
-            env.deposit(cell, m * (s0x - (double)nvx), m * (s0y - (double)nvy), m * (s0z - (double)nvz));
+            env.deposit(cell, sl, m * (s0x - (double)nvx), m * (s0y - (double)nvy), m * (s0z - (double)nvz));
             s0x = (double)nvx;
             s0y = (double)nvy;
             s0z = (double)nvz;
             cell = nc;
-            env.load_u(cell, ux, uy, uz);
+            sl = env.slot(cell);
+            env.load_u(cell, sl, ux, uy, uz);
         }
         vx = nvx;  // :1351-1352
         vy = nvy;
@@ -188,7 +204,7 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     st_off(s.u, off, vx);
     st_off(s.v, off, vy);
     st_off(s.w, off, vz);
-    env.deposit(cell, m * (s0x - (double)vx), m * (s0y - (double)vy), m * (s0z - (double)vz));
+    env.deposit(cell, sl, m * (s0x - (double)vx), m * (s0y - (double)vy), m * (s0z - (double)vz));
     box.add(px, py, pz);  // :1362
 }
 
@@ -211,15 +227,15 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
     Cell cell;
     locate<T, PERIODIC>(g, px, py, pz, I, cell);  // :1286
     env.enter(cell);
+    const typename Env::Slot sl = env.slot(cell);
     T ux, uy, uz;
-    env.load_u(cell, ux, uy, uz);  // :1287-1288
+    env.load_u(cell, sl, ux, uy, uz);  // :1287-1288
     prefetch();
 
     // :1295-1296
     const Quot<T> q(den);
-    T nvx = q(vx + f * ux);
-    T nvy = q(vy + f * uy);
-    T nvz = q(vz + f * uz);
+    T nvx, nvy, nvz;
+    q.three(vx + f * ux, vy + f * uy, vz + f * uz, nvx, nvy, nvz);
     T nx = px + nvx * ph.dt;
     T ny = py + nvy * ph.dt;
     T nz = pz + nvz * ph.dt;
@@ -264,7 +280,7 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
     st_off(s.u, off, nvx);
     st_off(s.v, off, nvy);
     st_off(s.w, off, nvz);
-    env.deposit(cell, m * (s0x - (double)nvx), m * (s0y - (double)nvy), m * (s0z - (double)nvz));
+    env.deposit(cell, sl, m * (s0x - (double)nvx), m * (s0y - (double)nvy), m * (s0z - (double)nvz));
     if (stats) box.add(nx, ny, nz);  // :1362
 }
 
@@ -525,7 +541,7 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
 }
 
 template <class T, bool PERIODIC, bool RHO>
-__global__ void __launch_bounds__(1024) k_pass_sorted(PassArgs<T> a) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4))) k_pass_sorted(PassArgs<T> a) {
     extern __shared__ double lds[];
     // Bricks in dispatch order: workgroups go round-robin over the XCDs, so the bricks
     // in flight at any time form one contiguous window of the parcel arrays (measured
