@@ -10,6 +10,10 @@ Reads the CSV files rocprofv3 writes (``--kernel-trace --stats`` gives
   process, the calibration factor of each counter (copy bytes / counter bytes), and the
   measured kernel's HBM traffic per launch with both factors applied.
 
+``--pmc`` adds a table of any other counters collected for the kernel (with VALU
+instructions per wave, and with ``--simds`` the VALU issue rate per SIMD against one
+wave64 instruction every 4 cycles at the peak clock).
+
 The last block is ``key: value`` lines that bench.py reads back (``traffic_bytes``
 and the workload keys it was measured on).
 
@@ -75,6 +79,10 @@ def main(argv=None):
     ap.add_argument("--title", default="rocprofv3 summary")
     ap.add_argument("--command", action="append", default=[], help="command line recorded in the summary")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE", help="workload key recorded")
+    ap.add_argument("--simds", type=int, default=0, help="SIMDs of the device (CUs x 4) for the VALU utilisation")
+    ap.add_argument("--clock-ghz", type=float, default=2.4, help="peak engine clock for the VALU issue bound")
+    ap.add_argument("--pmc", action="append", default=[],
+                    help="counter_collection.csv of a --pmc run: its counters for the kernel are tabulated")
     a = ap.parse_args(argv)
 
     w = sys.stdout.write
@@ -119,6 +127,30 @@ def main(argv=None):
               "algorithmic bytes)")
             keys["algorithmic_bytes"] = f"{a.algorithmic_bytes:.0f}"
         w(".\n")
+    if a.pmc:
+        per = {}
+        for path in a.pmc:
+            with open(path) as f:
+                names = sorted({r["Counter_Name"] for r in csv.DictReader(f)})
+            for c in names:
+                _, v = pick(counter_per_dispatch(path, c), a.kernel)
+                per[c] = mean(v)
+        w("\n## Other counters (per dispatch, mean over dispatches, summed over instances)\n\n")
+        w("| counter | value |\n|---|---|\n")
+        for c, v in per.items():
+            w(f"| {c} | {v:.4g} |\n")
+        if "SQ_INSTS_VALU" in per and "SQ_WAVES" in per:
+            w(f"\nVALU instructions per wave: {per['SQ_INSTS_VALU'] / per['SQ_WAVES']:.0f}.\n")
+            keys["valu_per_wave"] = f"{per['SQ_INSTS_VALU'] / per['SQ_WAVES']:.0f}"
+        if "SQ_INSTS_VALU" in per and a.simds:
+            # a wave64 VALU instruction holds a 16-lane SIMD for 4 cycles: at most
+            # clock / 4 instructions per SIMD per second over the traced duration
+            rate = per["SQ_INSTS_VALU"] / a.simds / (krow["avg_ns"] * 1e-9)
+            bound = a.clock_ghz * 1e9 / 4.0
+            w(f"VALU instructions per SIMD per microsecond over the traced {krow['avg_ns'] / 1e3:.1f} us: "
+              f"{rate / 1e6:.0f}, against {bound / 1e6:.0f} for one wave64 instruction every 4 cycles at "
+              f"{a.clock_ghz} GHz ({100 * rate / bound:.0f}%).\n")
+            keys["valu_issue_fraction"] = f"{rate / bound:.3f}"
     for kv in a.set:
         k, _, v = kv.partition("=")
         keys[k] = v
