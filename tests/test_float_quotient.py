@@ -74,3 +74,74 @@ def test_signed_zeros_and_exact_quotients():
         assert (np.signbit(got) == np.signbit(a / den)).all() and (got == 0).all()
     a = np.array([3.0, 9.0, -4.5, 5e6, 1.0, 49.0], dtype=np.float32)
     np.testing.assert_array_equal(via_reciprocal(a, den), a / den)
+
+
+def three(a3, den):
+    """The kernels' rule for the three components of a substep: the rounded products,
+    and the float division for a component whose product is below 2^-126 and whose
+    numerator is nonzero, taken only when the smallest product is below 2^-126."""
+    r = 1.0 / den.astype(np.float64)
+    out = [(a.astype(np.float64) * r).astype(np.float32) for a in a3]
+    small = np.minimum(np.minimum(np.abs(out[0]), np.abs(out[1])), np.abs(out[2])) < F32_MIN_NORMAL
+    with np.errstate(all="ignore"):
+        for k in range(3):
+            fix = small & (np.abs(out[k]) < F32_MIN_NORMAL) & (a3[k] != 0)
+            out[k] = np.where(fix, a3[k] / den, out[k])
+    return out
+
+
+def check_three(a3, den):
+    a3 = [a.astype(np.float32) for a in a3]
+    den = den.astype(np.float32)
+    got = three(a3, den)
+    with np.errstate(all="ignore"):
+        for k in range(3):
+            want = a3[k] / den
+            ok = np.isnan(want) | (got[k].view(np.uint32) == want.view(np.uint32))
+            assert ok.all(), (k, a3[k][~ok][:5], den[~ok][:5], got[k][~ok][:5], want[~ok][:5])
+
+
+def test_three_component_rule_against_ieee_division():
+    """Every result equals the IEEE float quotient, subnormal and zero results included,
+    with components mixed so the shared branch is taken for some and not others."""
+    rng = np.random.default_rng(23)
+    n = 2_000_000
+    den = (np.float32(1) + (10.0 ** rng.uniform(-4, 6, size=n)).astype(np.float32)).astype(np.float32)
+    a3 = []
+    for k in range(3):
+        kind = rng.integers(0, 5, size=n)
+        mag = np.where(kind == 0, 10.0 ** rng.uniform(-45, -36, size=n),       # subnormal-range quotients
+              np.where(kind == 1, 10.0 ** rng.uniform(-38, -30, size=n),       # near the normal threshold
+              np.where(kind == 2, 0.0, 10.0 ** rng.uniform(-8, 2, size=n))))   # zeros, velocities
+        a3.append((mag * rng.choice([-1.0, 1.0], size=n)).astype(np.float32))
+    check_three(a3, den)
+
+
+def test_three_component_rule_at_the_normal_threshold():
+    """Numerators placed so the products land within a few ulps of 2^-126, on both sides
+    (a product that rounds up to 2^-126 keeps the product path)."""
+    rng = np.random.default_rng(24)
+    n = 2_000_000
+    den = (rng.uniform(1.0, 4.0, size=n)).astype(np.float32)
+    t = 2.0 ** -126 * (1.0 + rng.integers(-40, 41, size=n) * 2.0 ** -24)
+    a = (t * den.astype(np.float64)).astype(np.float32)
+    zero = np.zeros(n, dtype=np.float32)
+    check_three([a, zero, -a], den)
+    check_three([a, np.full(n, 1.0, np.float32), a[::-1].copy()], den)
+
+
+def test_three_component_rule_on_exact_subnormal_midpoints():
+    """a / den exactly halfway between two subnormals with den = 2 D (D odd > 1): the
+    double product misses the midpoint by the rounding of 1 / den and, for these D (found
+    by a search over odd D < 400), rounds to the wrong neighbour for many j -- the
+    quotients the fallback exists for."""
+    j = np.arange(0, 4096, dtype=np.float64)
+    wrong = 0
+    for D in (49, 75, 77, 91, 93, 99, 117, 123):
+        den = np.full(j.size, 2.0 * D, dtype=np.float32)
+        a = ((2 * j + 1) * D * 2.0 ** -149).astype(np.float32)
+        assert np.all(a.astype(np.float64) == (2 * j + 1) * D * 2.0 ** -149)  # representable
+        r = 1.0 / den.astype(np.float64)
+        wrong += int(np.sum((a.astype(np.float64) * r).astype(np.float32) != a / den))
+        check_three([a, a[::-1].copy(), np.ones(j.size, np.float32)], den)
+    assert wrong > 1000  # the product path alone would be wrong on many of them

@@ -108,3 +108,45 @@ def test_wall_parcel_outside_mesh_is_refused_by_the_sort(ctx):
         stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, stk.params(method=stk.SORTED))
     chunk.close()
     field.close()
+
+
+@pytest.mark.parametrize("method", [stk.PLAIN, stk.SORTED])
+@pytest.mark.parametrize("boundary", [stk.WALL, stk.PERIODIC])
+def test_fp32_quotients_on_subnormal_midpoints(ctx, method, boundary):
+    """FP32 form with 1 + f = 98 exactly (mu 1, rho 18, d 1, one substep of 97 s) and a
+    carrier at rest: a parcel velocity (2j+1) 49 2^-149 makes v / (1 + f) an exact
+    midpoint between two float subnormals, where the double product alone rounds to the
+    wrong neighbour for many j.  Mixed with zeros and normal velocities, so the shared
+    test of the three components is taken for some parcels and not for others.  The
+    kernels must reproduce the oracle's IEEE float division bit for bit."""
+    rng = np.random.default_rng(41)
+    npc = 4096
+    n, o, e = (4, 4, 4), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    x, y, z = inputs.positions(41, npc, o, e, dtype=np.float32)
+
+    def comp():
+        j = rng.integers(0, 1 << 14, size=npc).astype(np.float64)
+        mid = (2 * j + 1) * 49 * 2.0 ** -149 * rng.choice([-1.0, 1.0], size=npc)
+        kind = rng.integers(0, 4, size=npc)
+        vals = np.where(kind == 0, 0.0, np.where(kind == 1, rng.normal(size=npc), mid))
+        return vals.astype(np.float32)
+
+    u, v, w = comp(), comp(), comp()
+    d = np.ones(npc, dtype=np.float32)
+    U = np.zeros((64, 3), dtype=np.float32)
+    case = inputs.Case(n, o, e, U, d, x, y, z, u, v, w, mu_c=1.0, rho_d=18.0, rho_c=18.0, dt_fluid=97.0, nsub=1,
+                       boundary=boundary)
+    assert np.float32(np.float32(18.0 * 1.0) * np.float32(97.0)) / np.float32(18.0) == np.float32(97.0)
+    chunk, field = cloud(ctx, case, stk.FP32)
+    stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, stk.params(boundary=boundary, method=method))
+    ref_src, _, ref = st_oracle.evolve_case(case, precision=st_oracle.FP32)
+    got = chunk.download()
+    for name, a, b in zip("dxyzuvw", got, ref):
+        bad = np.flatnonzero(a.view(np.uint32) != b.view(np.uint32))
+        assert bad.size == 0, (name, bad[:5], a[bad[:5]], b[bad[:5]])
+    src = field.download_source()
+    scale = max(np.abs(ref_src).max(), 1e-300)
+    assert np.abs(src - ref_src).max() <= 1e-12 * scale
+    assert np.any(got[4] != 0) and np.any(np.abs(got[4][got[4] != 0]) < 2.0 ** -126)  # subnormal results present
+    chunk.close()
+    field.close()
