@@ -45,8 +45,12 @@ __device__ __forceinline__ void load_ug(const double *ug, uint32_t c, double2 &r
     r2 = q[2];
 }
 
+// The periodic instances are held to 168 VGPRs, 3 waves per SIMD (their one spill is an
+// output address, stored before the loops and reloaded after them); the wall instances
+// need 188 and run at 2.
 template <bool PERIODIC, int B>
-__global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERIODIC ? 3 : 2)))
+k_adjoint_segment(AdjArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.np) return;
     const uint32_t n = a.np;
@@ -59,6 +63,7 @@ __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
     // compared with the derivative to rounding, not bit for bit); the replayed forward
     // keeps the forward's exact divisions, so it takes the forward's decisions.
     const double rden = 1.0 / den;
+    const Quot<double> qden(den);
 
     double xb0 = a.xb[p], xb1 = a.xb[n + p], xb2 = a.xb[2 * n + p];
     double vb0 = a.vb[p], vb1 = a.vb[n + p], vb2 = a.vb[2 * n + p];
@@ -83,7 +88,7 @@ __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
         for (; t <= t0; ++t) {
             double nx, ny, nz, nvx, nvy, nvz;
             Cell nc;
-            advance<double, PERIODIC>(a.g, a.ph, f, Quot<double>(den), ux, uy, uz, px, py, pz, vx, vy, vz, cell, part0, nx,
+            advance<double, PERIODIC>(a.g, a.ph, f, qden, ux, uy, uz, px, py, pz, vx, vy, vz, cell, part0, nx,
                                       ny, nz, nvx, nvy, nvz, nc);
             if (t != a.K && nc.c != cell.c) {
                 cell = nc;
@@ -109,7 +114,7 @@ __global__ void __launch_bounds__(256) k_adjoint_segment(AdjArgs a) {
                 rc[j] = cell.c;
                 double nx, ny, nz, nvx, nvy, nvz;
                 Cell nc;
-                const unsigned bnc = advance<double, PERIODIC>(a.g, a.ph, f, Quot<double>(den), ux, uy, uz, px, py, pz, vx, vy,
+                const unsigned bnc = advance<double, PERIODIC>(a.g, a.ph, f, qden, ux, uy, uz, px, py, pz, vx, vy,
                                                                vz, cell, part0, nx, ny, nz, nvx, nvy, nvz, nc);
                 rb |= bnc << (3 * j);
                 if (t != a.K && nc.c != cell.c) {

@@ -142,15 +142,30 @@ __device__ __forceinline__ void load_parcel(const Soa<T> &s, uint32_t p, T rho_c
 // both precision forms.  Three velocity components per substep share the denominator.
 template <class T>
 struct Quot;
+// For doubles, with r = RN(1 / den) and q0 = RN(a r) (within one ulp of a / den), the
+// residual t = q0 den - a is exact as an FMA and RN(q0 - t r) is RN(a / den)
+// (Markstein's theorem: r within half an ulp of 1 / den, q0 within one ulp, no under-
+// or overflow).  Written this way a zero numerator keeps its sign (t = +0, and
+// -0 + q0 = q0 for either zero).  A nonzero quotient below 2^-969 (where the residual
+// could underflow), or one above 2^1000 or NaN, sends all three components to the IEEE
+// division.  Per substep: 3 multiplies and 6 FMAs in place of three division sequences.
 template <>
 struct Quot<double> {
-    double den;
-    __device__ __forceinline__ explicit Quot(double d) : den(d) {}
+    double den, r;
+    __device__ __forceinline__ explicit Quot(double d) : den(d), r(1.0 / d) {}
+    __device__ __forceinline__ static bool tiny(double q) { return (fabs(q) < 0x1p-969) & (q != 0.0); }
     __device__ __forceinline__ void three(double ax, double ay, double az, double &rx, double &ry,
                                           double &rz) const {
-        rx = ax / den;
-        ry = ay / den;
-        rz = az / den;
+        const double qx = ax * r, qy = ay * r, qz = az * r;
+        rx = fma(-fma(qx, den, -ax), r, qx);
+        ry = fma(-fma(qy, den, -ay), r, qy);
+        rz = fma(-fma(qz, den, -az), r, qz);
+        const bool huge = !(fmax(fmax(fabs(qx), fabs(qy)), fabs(qz)) <= 0x1p1000);
+        if (tiny(qx) | tiny(qy) | tiny(qz) | huge) {
+            rx = ax / den;
+            ry = ay / den;
+            rz = az / den;
+        }
     }
 };
 // For floats, a / den == (float)((double)a * RN64(1 / den)) whenever the quotient is

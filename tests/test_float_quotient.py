@@ -145,3 +145,88 @@ def test_three_component_rule_on_exact_subnormal_midpoints():
         wrong += int(np.sum((a.astype(np.float64) * r).astype(np.float32) != a / den))
         check_three([a, a[::-1].copy(), np.ones(j.size, np.float32)], den)
     assert wrong > 1000  # the product path alone would be wrong on many of them
+
+
+# ---- FP64: one reciprocal per parcel and an FMA correction -------------------------
+# The FP64 kernels divide by 1 + f as q0 = RN(a r), r = RN(1 / den), then
+# RN(q0 + RN(a - q0 den) r) with both steps FMAs, falling back to the IEEE division for
+# all three components when a product is below 2^-969, zero, above 2^1000 or NaN.
+# numpy has no FMA: the FMA is emulated exactly with rationals (one rounding of the
+# exact x y + z, which float(Fraction) performs correctly).
+
+from fractions import Fraction
+import math
+
+
+def fma_exact(x, y, z):
+    """RN(x y + z) with one rounding; an exact zero takes IEEE's sign (the common sign
+    when x y and z are zeros of the same sign, +0 otherwise)."""
+    v = Fraction(x) * Fraction(y) + Fraction(z)
+    if v != 0:
+        return float(v)
+    p_neg = (math.copysign(1.0, x) < 0) != (math.copysign(1.0, y) < 0)
+    if x * y == 0.0 and z == 0.0 and p_neg == (math.copysign(1.0, z) < 0):
+        return math.copysign(0.0, z)
+    return 0.0
+
+
+def three64(a3, den):
+    """The kernels' rule: q0 = RN(a r), t = RN(q0 den - a), RN(q0 - t r), unless a
+    quotient is nonzero and below 2^-969, or above 2^1000."""
+    r = 1.0 / den
+    q = [a * r for a in a3]
+    tiny = any(qk != 0.0 and abs(qk) < 2.0 ** -969 for qk in q)
+    if tiny or max(abs(v) for v in q) > 2.0 ** 1000:
+        return [ak / den for ak in a3]
+    return [fma_exact(-fma_exact(qk, den, -ak), r, qk) for ak, qk in zip(a3, q)]
+
+
+def draw64(rng, kind):
+    if kind == 0:
+        return float(rng.uniform(-1, 1) * 10.0 ** rng.uniform(-320, -285))  # tiny and subnormal
+    if kind == 1:
+        return 0.0 if rng.random() < 0.5 else -0.0
+    if kind == 2:
+        return float(rng.uniform(-1, 1) * 10.0 ** rng.uniform(-20, 3))
+    return float(rng.uniform(-1, 1) * 10.0 ** rng.uniform(-280, 280))
+
+
+def near_midpoint64(rng, den):
+    """A numerator whose exact quotient lies within an ulp fraction of a rounding midpoint."""
+    q = float(rng.uniform(1, 2) * 2.0 ** int(rng.integers(-60, 10)))
+    return float(Fraction(q + math.ulp(q) / 2) * Fraction(den))
+
+
+def test_fp64_rule_against_ieee_division():
+    rng = np.random.default_rng(31)
+    bad = 0
+    for _ in range(20000):
+        den = 1.0 + float(10.0 ** rng.uniform(-8, 7))
+        a3 = [draw64(rng, int(rng.integers(0, 4))) if rng.random() < 0.6 else near_midpoint64(rng, den)
+              for _ in range(3)]
+        got = three64(a3, den)
+        for ak, gk in zip(a3, got):
+            want = ak / den
+            bad += np.float64(gk).view(np.uint64) != np.float64(want).view(np.uint64)
+    assert bad == 0
+
+
+def test_fp64_rule_keeps_signed_zeros():
+    """Zero numerators stay on the FMA path (no fallback) and keep their sign."""
+    for den in (1.0, 1.5, 98.0, 1.0 + 2.0 ** -40, 3.7e5):
+        for a in (0.0, -0.0):
+            got = three64([a, 1.0, -2.5], den)
+            assert math.copysign(1.0, got[0]) == math.copysign(1.0, a) and got[0] == 0.0
+            assert got[1] == 1.0 / den and got[2] == -2.5 / den
+
+
+def test_fp64_correction_is_needed():
+    """The uncorrected product RN(a r) misses the IEEE quotient on a large share of
+    near-midpoint numerators, so the agreement above pins the FMA step."""
+    rng = np.random.default_rng(32)
+    wrong = 0
+    for _ in range(4000):
+        den = 1.0 + float(10.0 ** rng.uniform(-8, 7))
+        a = near_midpoint64(rng, den)
+        wrong += (a * (1.0 / den)) != a / den
+    assert wrong > 400

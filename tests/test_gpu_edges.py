@@ -150,3 +150,54 @@ def test_fp32_quotients_on_subnormal_midpoints(ctx, method, boundary):
     assert np.any(got[4] != 0) and np.any(np.abs(got[4][got[4] != 0]) < 2.0 ** -126)  # subnormal results present
     chunk.close()
     field.close()
+
+
+@pytest.mark.parametrize("method", [stk.PLAIN, stk.SORTED])
+@pytest.mark.parametrize("fused", [1, 4])
+def test_fp64_quotients_tiny_zero_and_near_midpoints(ctx, method, fused):
+    """FP64 with 1 + f = 98 exactly (mu 97, rho 18, d 1, substeps of 1 s) and a carrier
+    at rest, so the drag numerator is the parcel velocity: velocities below 2^-969 (the
+    fallback), zero, ordinary, or placed so v / 98 lies next to a rounding midpoint.  The
+    reciprocal-and-FMA quotient and its fallback must reproduce the oracle's divisions
+    bit for bit over four substeps.  Quotients stay above the double subnormal range
+    (2.2e-308): there the GPU's FP64 division itself can differ from IEEE in the last
+    place."""
+    from fractions import Fraction
+    import math
+
+    rng = np.random.default_rng(43)
+    npc = 4096
+    n, o, e = (4, 4, 4), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    x, y, z = inputs.positions(43, npc, o, e, dtype=np.float64)
+
+    def comp():
+        out = np.empty(npc)
+        for i in range(npc):
+            k = rng.integers(0, 5)
+            if k == 0:
+                out[i] = rng.uniform(-1, 1) * 10.0 ** rng.uniform(-295, -290)  # below 2^-969
+            elif k == 1:
+                out[i] = 0.0
+            elif k == 2:
+                out[i] = rng.normal()
+            else:
+                q = float(rng.uniform(1, 2) * 2.0 ** int(rng.integers(-40, -5)))
+                out[i] = float(Fraction(q + math.ulp(q) / 2) * 98)
+        return out
+
+    u, v, w = comp(), comp(), comp()
+    d = np.ones(npc)
+    U = np.zeros((64, 3))
+    case = inputs.Case(n, o, e, U, d, x, y, z, u, v, w, mu_c=97.0, rho_d=18.0, rho_c=18.0, dt_fluid=4.0,
+                       nsub=4, boundary=stk.PERIODIC)
+    chunk, field = cloud(ctx, case, stk.FP64)
+    stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub,
+                      stk.params(boundary=stk.PERIODIC, method=method, substeps_per_pass=fused))
+    ref_src, _, ref = st_oracle.evolve_case(case, precision=st_oracle.FP64)
+    for name, a, b in zip("dxyzuvw", chunk.download(), ref):
+        bad = np.flatnonzero(a.view(np.uint64) != b.view(np.uint64))
+        assert bad.size == 0, (name, bad[:5], a[bad[:5]], b[bad[:5]])
+    src = field.download_source()
+    assert np.abs(src - ref_src).max() <= 1e-12 * max(np.abs(ref_src).max(), 1e-300)
+    chunk.close()
+    field.close()
