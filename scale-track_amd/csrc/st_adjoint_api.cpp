@@ -31,6 +31,7 @@ struct st_adjoint {
     double *mbar = nullptr;  // n
     double *G = nullptr;     // 3 * ncell
     double *UG = nullptr;    // 6 * ncell: U and G of each cell side by side (reverse sweep)
+    double *T = nullptr;     // 3 * ncell: target source of st_adjoint_source_misfit
     double *tmp = nullptr;   // 6n staging
     double *scal = nullptr;  // [0] loss, [1] d L / d cd
 };
@@ -40,7 +41,7 @@ namespace {
 int destroy(st_adjoint *a) {
     if (!a) return ST_OK;
     quiesce_ctx(a->ctx);
-    for (double *q : {a->ck, a->xb, a->vb, a->vseed, a->fbar, a->mbar, a->G, a->UG, a->tmp, a->scal})
+    for (double *q : {a->ck, a->xb, a->vb, a->vseed, a->fbar, a->mbar, a->G, a->UG, a->T, a->tmp, a->scal})
         if (q) (void)hipFree(q);
     st_ctx *ctx = a->ctx;
     delete a;
@@ -88,6 +89,7 @@ int st_adjoint_create(st_ctx *ctx, st_cloud *c, st_field *f, int32_t nsub, int32
     if (e == hipSuccess) e = hipMalloc((void **)&a->mbar, n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->G, nc3 * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->UG, 2 * nc3 * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->T, nc3 * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->tmp, 6 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->scal, 2 * 8);
     if (e == hipSuccess) e = st::launch_fill_f64(a->G, 0.0, nc3, ctx->stream);
@@ -136,14 +138,11 @@ int st_adjoint_source_misfit(st_adjoint *a, const double *target, double *loss) 
     hipStream_t s = a->ctx->stream;
     const size_t nc3 = (size_t)f->ncell * 3;
     ST_TRY(merge_source(f));
-    double *T = nullptr;
-    ST_HIP(hipMalloc((void **)&T, nc3 * 8));
-    hipError_t e = hipMemcpyAsync(T, target, nc3 * 8, hipMemcpyHostToDevice, s);
+    hipError_t e = hipMemcpyAsync(a->T, target, nc3 * 8, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = st::launch_fill_f64(a->scal, 0.0, 1, s);
-    if (e == hipSuccess) e = st::launch_misfit(f->src, T, a->G, a->scal, nc3, s);
+    if (e == hipSuccess) e = st::launch_misfit(f->src, a->T, a->G, a->scal, nc3, s);
     if (e == hipSuccess) e = hipMemcpyAsync(loss, a->scal, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(T);
     if (e != hipSuccess) return fail(ST_E_DEVICE, "source misfit: %s", hipGetErrorString(e));
     return ST_OK;
 }
