@@ -1,0 +1,95 @@
+"""Randomised configurations of the GPU paths against the oracle.
+
+Each seed draws a whole configuration:
+- mesh dims 1..40 per axis, a non-cubic box at an arbitrary origin;
+- a divergence-free carrier of random amplitude;
+- 1..30000 parcels with random diameters and velocities (per-parcel density or not);
+- random carrier properties and c_D multiplier;
+- 1..12 substeps in passes of random length;
+- boundary, precision form, method, brick shape, block size and flush mode.
+
+Two calls are made per configuration, the source zeroed in between.  The bars are
+those of the parity tests: state and bounding box bit-exact; the source within 1e-12
+of its largest magnitude.  The seeds are fixed, so a failure names its configuration.
+"""
+import numpy as np
+import pytest
+
+import scaletrack as stk
+from scaletrack import inputs
+from oracle import st_oracle
+
+pytestmark = pytest.mark.gpu
+
+LDS_LIMIT = 65536
+
+
+def draw(seed):
+    rng = np.random.default_rng(1000 + seed)
+    precision = stk.FP32 if rng.random() < 0.5 else stk.FP64
+    dt = np.float32 if precision == stk.FP32 else np.float64
+    boundary = stk.PERIODIC if rng.random() < 0.5 else stk.WALL
+    n = tuple(int(v) for v in rng.integers(1, 41, size=3))
+    if rng.random() < 0.25:
+        n = (n[0], n[1], 1)  # a 2D mesh
+    origin = tuple(float(v) for v in rng.uniform(-2.0, 2.0, size=3))
+    ending = tuple(o + float(L) for o, L in zip(origin, rng.uniform(0.05, 3.0, size=3)))
+    U = inputs.fourier_field(n, origin, ending, modes=6, seed=seed, amplitude=float(10 ** rng.uniform(-1, 1)),
+                             dtype=dt)
+    big = int(rng.integers(100, 30001))
+    npc = int(rng.choice([1, 2, 63, 64, 65, big, big, big, big, big]))
+    x, y, z = inputs.positions(seed, npc, origin, ending, dtype=dt)
+    d = inputs.lognormal_diameters(seed, npc, median=float(10 ** rng.uniform(-5.3, -3.7)), dtype=dt)
+    vs = float(10 ** rng.uniform(-2, 0.5))
+    u, v, w = [(rng.standard_normal(npc) * vs).astype(dt) for _ in range(3)]
+    rho_d = float(rng.uniform(500.0, 5000.0))
+    rho = (rho_d * rng.uniform(0.5, 2.0, size=npc)).astype(dt) if rng.random() < 0.4 else None
+    nsub = int(rng.integers(1, 13))
+    case = inputs.Case(n, origin, ending, U, d, x, y, z, u, v, w, rho=rho, mu_c=float(10 ** rng.uniform(-5, -2)),
+                       rho_d=rho_d, rho_c=float(rng.uniform(1.0, 1000.0)),
+                       dt_fluid=float(10 ** rng.uniform(-4, -2)), nsub=nsub, boundary=boundary,
+                       name=f"sweep {seed}")
+    method = stk.SORTED if rng.random() < 0.7 else stk.PLAIN
+    per = int(rng.choice([0, 1, 2, 3, nsub]))
+    elem = 8 if precision == stk.FP64 else 4
+    while True:
+        brick = tuple(int(v) for v in rng.integers(1, 17, size=3))
+        tile = (brick[0] + 2) * (brick[1] + 2) * (brick[2] + 2)
+        if brick[0] * brick[1] * brick[2] <= 2048 and tile * 3 * (8 + elem) <= LDS_LIMIT:
+            break
+    p = stk.params(boundary=boundary, method=method, substeps_per_pass=per, brick=brick,
+                   block=int(rng.choice([64, 128, 256, 512, 1024])),
+                   flush=stk.FLUSH_OWNER if rng.random() < 0.7 else stk.FLUSH_ATOMIC,
+                   cd=float(rng.uniform(0.5, 2.0)))
+    return case, precision, p
+
+
+@pytest.mark.parametrize("seed", range(96))
+def test_random_configuration(ctx, seed):
+    case, precision, p = draw(seed)
+    ident = (f"seed {seed}: n={case.n} np={case.np} {'fp32' if precision == stk.FP32 else 'fp64'} "
+             f"{'periodic' if case.boundary else 'wall'} method={p.method} brick={tuple(p.brick)} "
+             f"block={p.block} flush={p.flush} nsub={case.nsub} per={p.substeps_per_pass} "
+             f"rho={'parcel' if case.rho is not None else 'chunk'}")
+    field = stk.TwoWayEulerian(ctx, stk.Mesh(case.n, case.origin, case.ending), precision)
+    field.upload_U(case.U)
+    chunk = stk.Chunk(ctx, case.np, case.mu_c, case.rho_d, case.rho_d_by_rho_c, precision, rho_p=case.rho,
+                      rho_c=case.rho_c)
+    chunk.set_state(*case.state())
+    chunk.upload()
+    st = case.state()
+    oprec = st_oracle.FP32 if precision == stk.FP32 else st_oracle.FP64
+    for call in range(2):
+        field.zero_source()
+        bb = stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p)
+        src = field.download_source()
+        ref_src, ref_bb, st = st_oracle.evolve_case(case, state=st, precision=oprec,
+                                                   substeps_per_pass=p.substeps_per_pass, cd=p.cd)
+        scale = max(np.abs(ref_src).max(), 1e-300)
+        assert np.abs(src - ref_src).max() <= 1e-12 * scale, f"{ident}, call {call}: source"
+        np.testing.assert_array_equal(np.concatenate(bb), ref_bb, err_msg=f"{ident}, call {call}: bbox")
+    for name, a, b in zip("dxyzuvw", chunk.download(), st):
+        bad = np.flatnonzero(a != b)
+        assert bad.size == 0, f"{ident}: {name} differs for {bad.size} parcels, first {bad[:3]}"
+    chunk.close()
+    field.close()
