@@ -8,6 +8,10 @@ Each seed draws a whole configuration:
 - 1..12 substeps in passes of random length;
 - boundary, precision form, method, brick shape, block size and flush mode.
 
+The batch sweep draws 1..6 clouds per seed (empty, one-parcel and large clouds, with and
+without a per-parcel density, each on its own random mesh) with a random brick target,
+per-CU cap, block, flush mode and substeps per pass, and checks every cloud.
+
 Two calls are made per configuration, the source zeroed in between.  The bars are
 those of the parity tests: state and bounding box bit-exact; the source within 1e-12
 of its largest magnitude.  The seeds are fixed, so a failure names its configuration.
@@ -93,3 +97,69 @@ def test_random_configuration(ctx, seed):
         assert bad.size == 0, f"{ident}: {name} differs for {bad.size} parcels, first {bad[:3]}"
     chunk.close()
     field.close()
+
+
+def draw_batch(seed):
+    """1..6 clouds sharing precision and params, each with its own random mesh, cloud and
+    density form; random brick target, block, per-CU cap and substeps per pass."""
+    rng = np.random.default_rng(5000 + seed)
+    precision = stk.FP32 if rng.random() < 0.5 else stk.FP64
+    dt = np.float32 if precision == stk.FP32 else np.float64
+    boundary = stk.PERIODIC if rng.random() < 0.5 else stk.WALL
+    cases = []
+    for i in range(int(rng.integers(1, 7))):
+        n = tuple(int(v) for v in rng.integers(1, 33, size=3))
+        origin = tuple(float(v) for v in rng.uniform(-1.0, 1.0, size=3))
+        ending = tuple(o + float(L) for o, L in zip(origin, rng.uniform(0.1, 2.0, size=3)))
+        U = inputs.fourier_field(n, origin, ending, modes=5, seed=100 * seed + i,
+                                 amplitude=float(10 ** rng.uniform(-1, 0.7)), dtype=dt)
+        npc = int(rng.choice([0, 1, 65, int(rng.integers(100, 20001)), int(rng.integers(100, 20001))]))
+        x, y, z = inputs.positions(100 * seed + i, npc, origin, ending, dtype=dt)
+        d = inputs.lognormal_diameters(100 * seed + i, npc, median=float(10 ** rng.uniform(-5, -3.8)), dtype=dt)
+        u, v, w = [(rng.standard_normal(npc) * 0.3).astype(dt) for _ in range(3)]
+        rho = (1500.0 * rng.uniform(0.5, 2.0, size=npc)).astype(dt) if rng.random() < 0.3 else None
+        cases.append(inputs.Case(n, origin, ending, U, d, x, y, z, u, v, w, rho=rho, mu_c=1.8e-5, rho_d=1500.0,
+                                 rho_c=1.2, dt_fluid=2e-3, nsub=5, boundary=boundary))
+    p = stk.params(boundary=boundary, substeps_per_pass=int(rng.choice([0, 1, 2])),
+                   block=int(rng.choice([128, 256, 512, 1024])),
+                   flush=stk.FLUSH_OWNER if rng.random() < 0.7 else stk.FLUSH_ATOMIC)
+    target = int(rng.choice([0, 64, 512, 4096]))
+    per_cu = int(rng.choice([0, 1, 2, 3, 4]))
+    return cases, precision, p, target, per_cu
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_batch(ctx, seed):
+    cases, precision, p, target, per_cu = draw_batch(seed)
+    ident = (f"batch seed {seed}: {len(cases)} clouds, nps={[c.np for c in cases]}, "
+             f"{'fp32' if precision == stk.FP32 else 'fp64'} target={target} per_cu={per_cu} "
+             f"block={p.block} flush={p.flush} per={p.substeps_per_pass}")
+    objs = []
+    for c in cases:
+        field = stk.TwoWayEulerian(ctx, stk.Mesh(c.n, c.origin, c.ending), precision)
+        field.upload_U(c.U)
+        chunk = stk.Chunk(ctx, c.np, c.mu_c, c.rho_d, c.rho_d_by_rho_c, precision, rho_p=c.rho, rho_c=c.rho_c)
+        chunk.set_state(*c.state())
+        chunk.upload()
+        objs.append((chunk, field))
+    batch = stk.Batch(ctx, [o[0] for o in objs], [o[1] for o in objs], p, target_parcels_per_brick=target,
+                      blocks_per_cu=per_cu)
+    states = [c.state() for c in cases]
+    oprec = st_oracle.FP32 if precision == stk.FP32 else st_oracle.FP64
+    for call in range(2):
+        for _, f in objs:
+            f.zero_source()
+        batch.evolve(cases[0].dt_fluid, cases[0].nsub)
+        for i, (c, (ch, f)) in enumerate(zip(cases, objs)):
+            ref_src, _, states[i] = st_oracle.evolve_case(c, state=states[i], precision=oprec,
+                                                          substeps_per_pass=p.substeps_per_pass)
+            src = f.download_source()
+            scale = max(np.abs(ref_src).max(), 1e-300)
+            assert np.abs(src - ref_src).max() <= 1e-12 * scale, f"{ident}: cloud {i} call {call} source"
+    for i, (ch, _) in enumerate(objs):
+        for name, a, b in zip("dxyzuvw", ch.download(), states[i]):
+            assert np.array_equal(a, b), f"{ident}: cloud {i} {name} differs"
+    batch.close()
+    for ch, f in objs:
+        ch.close()
+        f.close()
