@@ -146,6 +146,10 @@ int validate_params(const st_params &p) {
     return ST_OK;
 }
 
+// Brick counts up to this sort through per-workgroup LDS histograms (32 KB of LDS);
+// larger ones through one global counter per brick.
+constexpr int64_t kSortLdsBins = 8192;
+
 template <class T>
 int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
     hipStream_t s = c->ctx->stream;
@@ -171,13 +175,29 @@ int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
     if (!c->keys) ST_HIP(hipMalloc((void **)&c->keys, (size_t)(c->n > 0 ? c->n : 1) * 4));
     ST_TRY(alloc_set(c, 1 - c->cur));
     const uint32_t np = (uint32_t)c->n;
-    ST_HIP(launch_fill_u32(c->counts, 0u, (size_t)nbr, s));
     ST_HIP(launch_fill_u32(c->bad, 0u, 1, s));
-    ST_HIP(launch_brick_keys<T>(soa<T>(c, c->cur), np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC,
-                                    c->keys, c->counts, c->bad, s));
-    ST_HIP(launch_scan(c->counts, (int32_t)nbr, c->brick_start, c->cursor, s));
-    ST_HIP(launch_scatter<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur),
-                                 c->pid[1 - c->cur], c->keys, c->cursor, np, s));
+    if (nbr <= kSortLdsBins) {
+        // chunked: one histogram row per workgroup of at least 4096 parcels, at most 512 rows
+        const int32_t rows = (int32_t)std::max<int64_t>(1, std::min<int64_t>(512, ((int64_t)np + 4095) / 4096));
+        const size_t need = (size_t)rows * (size_t)nbr;
+        if (need > c->hist_cap) {
+            if (c->hist) ST_HIP(hipFree(c->hist));
+            c->hist = nullptr;
+            c->hist_cap = 0;
+            ST_HIP(hipMalloc((void **)&c->hist, need * 4));
+            c->hist_cap = need;
+        }
+        ST_HIP(launch_sort_chunked<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur), c->pid[1 - c->cur],
+                                      np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC, rows, c->keys, c->hist,
+                                      c->counts, c->brick_start, c->cursor, c->bad, s));
+    } else {
+        ST_HIP(launch_fill_u32(c->counts, 0u, (size_t)nbr, s));
+        ST_HIP(launch_brick_keys<T>(soa<T>(c, c->cur), np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC,
+                                        c->keys, c->counts, c->bad, s));
+        ST_HIP(launch_scan(c->counts, (int32_t)nbr, c->brick_start, c->cursor, s));
+        ST_HIP(launch_scatter<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur),
+                                     c->pid[1 - c->cur], c->keys, c->cursor, np, s));
+    }
     // Copy the sorted order back so the cloud's device pointers never change: graphs
     // and batch tables captured earlier stay valid across re-sorts.
     const size_t bytes = (size_t)c->n * c->es;
@@ -494,6 +514,7 @@ int st_cloud_destroy(st_cloud *c) {
     if (c->keys) hipFree(c->keys);
     if (c->counts) hipFree(c->counts);
     if (c->cursor) hipFree(c->cursor);
+    if (c->hist) hipFree(c->hist);
     if (c->brick_start) hipFree(c->brick_start);
     if (c->bbox) hipFree(c->bbox);
     if (c->part) hipFree(c->part);

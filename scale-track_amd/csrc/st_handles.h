@@ -69,6 +69,8 @@ struct st_cloud {
     uint32_t *keys = nullptr;
     uint32_t *counts = nullptr;
     uint32_t *cursor = nullptr;
+    uint32_t *hist = nullptr;            // per-chunk brick histograms of the chunked sort
+    size_t hist_cap = 0;                 // elements
     int32_t *brick_start = nullptr;
     int32_t bins_cap = 0;
     bool sorted = false;

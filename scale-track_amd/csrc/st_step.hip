@@ -771,23 +771,32 @@ hipError_t launch_merge(double *src, double *stray, size_t n, hipStream_t s) {
 // Counting sort of parcels by brick.
 
 // Wave-aggregated atomicAdd: lanes with equal keys share one atomic; each active lane
-// gets base + its rank among them.  All lanes of the wave must call it.
+// gets base + its rank among them (lanes of a group in lane order).  The groups are
+// found first with ballots alone; then every group's first lane adds its group's size
+// in one atomic instruction for the whole wave, so a wave whose 64 keys all differ
+// (an unsorted cloud) waits for one memory round trip, not 64 in a row.  All lanes of
+// the wave must call it.
 __device__ __forceinline__ uint32_t wave_claim(uint32_t *ctr, uint32_t key, bool active) {
     const int lane = threadIdx.x % kWave;
     unsigned long long pending = __ballot(active);
-    uint32_t result = 0;
+    int my_leader = lane;
+    uint32_t rank = 0, group = 0;
     while (pending) {
         const int leader = __ffsll((long long)pending) - 1;
         const uint32_t lkey = __shfl(key, leader, kWave);
         const bool mine = ((pending >> lane) & 1ull) && key == lkey;
         const unsigned long long grp = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(ctr + lkey, (uint32_t)__popcll(grp));
-        base = __shfl(base, leader, kWave);
-        if (mine) result = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+        if (mine) {
+            my_leader = leader;
+            rank = (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+        }
+        if (lane == leader) group = (uint32_t)__popcll(grp);
         pending &= ~grp;
     }
-    return result;
+    uint32_t base = 0;
+    if (group) base = atomicAdd(ctr + key, group);  // a leader's own key is its group's key
+    base = __shfl(base, my_leader, kWave);
+    return base + rank;
 }
 
 template <class T, bool PERIODIC>
@@ -863,6 +872,141 @@ hipError_t launch_scan(const uint32_t *counts, int32_t nbins, int32_t *start, ui
     k_scan<<<1, 1024, 0, s>>>(counts, nbins, start, cursor);
     return hipGetLastError();
 }
+
+// Chunked counting sort, for brick counts whose histogram fits one workgroup's LDS.
+// Workgroup g owns parcels [g C, (g+1) C): it counts its keys in LDS and writes its
+// row of hist (g-major, nbins per row); k_col_prefix turns every column (brick) into
+// the exclusive prefix over workgroups and the brick totals; the scatter then claims
+// slots with LDS atomics on start[b] + hist[g][b].  No global atomics at all, where
+// one counter per brick would take every parcel's atomic.
+constexpr int kSortBlock = 1024;
+
+template <class T, bool PERIODIC>
+__global__ void __launch_bounds__(kSortBlock) k_sort_count(Soa<T> s, uint32_t np, uint32_t chunk, Geo g,
+                                                           int32_t nb0, int32_t nb1, int32_t bd0, int32_t bd1,
+                                                           int32_t bd2, int32_t nbins, uint32_t *keys,
+                                                           uint32_t *hist, unsigned int *bad) {
+    extern __shared__ uint32_t h[];
+    for (int32_t b = threadIdx.x; b < nbins; b += kSortBlock) h[b] = 0u;
+    __syncthreads();
+    const uint32_t p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, np);
+    uint32_t nbad = 0;
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += kSortBlock) {
+        int32_t I[3];
+        Cell cl;
+        locate<T, PERIODIC>(g, s.x[p], s.y[p], s.z[p], I, cl);
+        if (!PERIODIC && cl.part != 0) ++nbad;
+        const uint32_t key = (uint32_t)(cl.f[0] / bd0) +
+                             (uint32_t)nb0 * ((uint32_t)(cl.f[1] / bd1) + (uint32_t)nb1 * (uint32_t)(cl.f[2] / bd2));
+        keys[p] = key;
+        atomicAdd(&h[key], 1u);
+    }
+    if (nbad) atomicAdd(bad, nbad);
+    __syncthreads();
+    uint32_t *row = hist + (size_t)blockIdx.x * nbins;
+    for (int32_t b = threadIdx.x; b < nbins; b += kSortBlock) row[b] = h[b];
+}
+
+// Per brick: the exclusive prefix of its counts over the workgroups (in place) and the
+// total (into counts, for k_scan).
+__global__ void k_col_prefix(uint32_t *__restrict__ hist, int32_t nbins, int32_t nrows, uint32_t *counts) {
+    const int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    constexpr int kBatch = 16;  // loads in flight per thread
+    uint32_t run = 0;
+    for (int32_t r0 = 0; r0 < nrows; r0 += kBatch) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) v[k] = r0 + k < nrows ? hist[(size_t)(r0 + k) * nbins + b] : 0u;
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            if (r0 + k < nrows) hist[(size_t)(r0 + k) * nbins + b] = run;
+            run += v[k];
+        }
+    }
+    counts[b] = run;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kSortBlock) k_sort_scatter(Soa<T> in, const int32_t *pid_in, Soa<T> out,
+                                                             int32_t *pid_out, const uint32_t *keys,
+                                                             const uint32_t *hist, const int32_t *start,
+                                                             int32_t nbins, uint32_t np, uint32_t chunk) {
+    extern __shared__ uint32_t cur[];
+    const uint32_t *row = hist + (size_t)blockIdx.x * nbins;
+    for (int32_t b = threadIdx.x; b < nbins; b += kSortBlock) cur[b] = (uint32_t)start[b] + row[b];
+    __syncthreads();
+    const uint32_t p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, np);
+    // kUnroll parcels per thread per round, every load issued before the first store
+    // (two for doubles, which keeps two 1024-thread workgroups per CU)
+    constexpr int kUnroll = sizeof(T) == 4 ? 4 : 2;
+    for (uint32_t pb = p0 + threadIdx.x; pb < p1; pb += kUnroll * kSortBlock) {
+        uint32_t q[kUnroll];
+        T d[kUnroll], x[kUnroll], y[kUnroll], z[kUnroll], u[kUnroll], v[kUnroll], w[kUnroll], r[kUnroll];
+        int32_t id[kUnroll];
+        bool on[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+            const uint32_t p = pb + k * kSortBlock;
+            on[k] = p < p1;
+            const uint32_t pc = on[k] ? p : p0;
+            q[k] = keys[pc];
+            d[k] = in.d[pc];
+            x[k] = in.x[pc];
+            y[k] = in.y[pc];
+            z[k] = in.z[pc];
+            u[k] = in.u[pc];
+            v[k] = in.v[pc];
+            w[k] = in.w[pc];
+            r[k] = in.rho ? in.rho[pc] : (T)0;
+            id[k] = pid_in[pc];
+        }
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+            if (!on[k]) continue;
+            const uint32_t o = atomicAdd(&cur[q[k]], 1u);
+            out.d[o] = d[k];
+            out.x[o] = x[k];
+            out.y[o] = y[k];
+            out.z[o] = z[k];
+            out.u[o] = u[k];
+            out.v[o] = v[k];
+            out.w[o] = w[k];
+            if (in.rho) ((T *)out.rho)[o] = r[k];
+            pid_out[o] = id[k];
+        }
+    }
+}
+
+template <class T>
+hipError_t launch_sort_chunked(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out, uint32_t np,
+                               Geo g, const int32_t nb[3], const int32_t bd[3], int periodic, int32_t nrows,
+                               uint32_t *keys, uint32_t *hist, uint32_t *counts, int32_t *start,
+                               uint32_t *cursor, unsigned int *bad, hipStream_t s) {
+    const int32_t nbins = nb[0] * nb[1] * nb[2];
+    const uint32_t chunk = (np + (uint32_t)nrows - 1) / (uint32_t)nrows;
+    const size_t lds = (size_t)nbins * 4;
+    if (periodic)
+        k_sort_count<T, true><<<nrows, kSortBlock, lds, s>>>(in, np, chunk, g, nb[0], nb[1], bd[0], bd[1], bd[2],
+                                                             nbins, keys, hist, bad);
+    else
+        k_sort_count<T, false><<<nrows, kSortBlock, lds, s>>>(in, np, chunk, g, nb[0], nb[1], bd[0], bd[1], bd[2],
+                                                              nbins, keys, hist, bad);
+    k_col_prefix<<<grid_for((size_t)nbins, 256), 256, 0, s>>>(hist, nbins, nrows, counts);
+    k_scan<<<1, 1024, 0, s>>>(counts, nbins, start, cursor);
+    if (np) k_sort_scatter<T><<<nrows, kSortBlock, lds, s>>>(in, pid_in, out, pid_out, keys, hist, start, nbins, np,
+                                                             chunk);
+    return hipGetLastError();
+}
+template hipError_t launch_sort_chunked<float>(const Soa<float> &, const int32_t *, Soa<float>, int32_t *, uint32_t,
+                                               Geo, const int32_t[3], const int32_t[3], int, int32_t, uint32_t *,
+                                               uint32_t *, uint32_t *, int32_t *, uint32_t *, unsigned int *,
+                                               hipStream_t);
+template hipError_t launch_sort_chunked<double>(const Soa<double> &, const int32_t *, Soa<double>, int32_t *,
+                                                uint32_t, Geo, const int32_t[3], const int32_t[3], int, int32_t,
+                                                uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *,
+                                                unsigned int *, hipStream_t);
+
 
 template <class T>
 __global__ void k_scatter(Soa<T> in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out,
