@@ -115,7 +115,7 @@ def test_adjoint_endpoint_loss(ctx, boundary):
 
 @pytest.mark.parametrize("boundary", [0, 1])
 def test_adjoint_vs_frozen_event_fd(ctx, boundary):
-    """The north-star bar: adjoint vs central differences (eps = 1e-6) within 1e-4."""
+    """The accuracy bar of the adjoint: central differences (eps = 1e-6) agree within 1e-4."""
     case = make_case(boundary=boundary, np_parcels=2000)
     T = target(case)
     field, chunk = gpu_objects(ctx, case)
