@@ -116,11 +116,17 @@ int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
     a->dt_sub = substep_dt(ST_FP64, dt_fluid, a->nsub);
     const uint32_t n = a->n;
     for (int32_t seg = 0; seg < a->nseg; ++seg) {
-        st::Soa<double> cur = soa<double>(c, c->cur);
-        const double *fields[6] = {cur.x, cur.y, cur.z, cur.u, cur.v, cur.w};
-        ST_HIP(st::launch_save6(fields, a->ck + (size_t)seg * 6 * n, n, s));
+        double *ck = a->ck + (size_t)seg * 6 * n;
         const int32_t ks = std::min(a->K, a->nsub - seg * a->K);
-        ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false));
+        if (p.method == ST_METHOD_SORTED) {
+            // the segment's first pass writes the checkpoint from the state it loads
+            ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false, ck));
+        } else {
+            st::Soa<double> cur = soa<double>(c, c->cur);
+            const double *fields[6] = {cur.x, cur.y, cur.z, cur.u, cur.v, cur.w};
+            ST_HIP(st::launch_save6(fields, ck, n, s));
+            ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false));
+        }
     }
     ST_HIP(st::launch_fill_f64(a->xb, 0.0, 3 * (size_t)n, s));
     ST_HIP(st::launch_fill_f64(a->vseed, 0.0, 3 * (size_t)n, s));

@@ -248,7 +248,7 @@ int range_check(st_cloud *c, st_field *f) {
 
 template <class T>
 int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p,
-               bool capturing) {
+               bool capturing, T *ck) {
     hipStream_t s = ctx->stream;
     PassArgs<T> a{};
     a.s = soa<T>(c, c->cur);
@@ -278,6 +278,7 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     for (int done = 0; done < nsub; done += per) {
         a.nsteps = per < nsub - done ? per : nsub - done;
         a.part = done + a.nsteps >= nsub ? c->part : nullptr;
+        a.ck = done == 0 ? ck : nullptr;
         if (p.method == ST_METHOD_SORTED)
             ST_HIP(launch_pass_sorted<T>(a, periodic, p.block, s));
         else
@@ -371,8 +372,8 @@ Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd
 }
 template Phys<float> make_phys<float>(const st_cloud *, const st_field *, double, double);
 template Phys<double> make_phys<double>(const st_cloud *, const st_field *, double, double);
-template int run_passes<float>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool);
-template int run_passes<double>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool);
+template int run_passes<float>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool, float *);
+template int run_passes<double>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool, double *);
 
 }  // namespace host
 }  // namespace st

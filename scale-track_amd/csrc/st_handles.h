@@ -131,9 +131,11 @@ bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p);
 int sort_any(st_cloud *c, st_field *f, const st_params &p);
 int prepare(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int nsub, const st_params *pin, st_params &p);
 // Queues nsub substeps of length dt_sub (exact in T) on the context stream, split in
-// passes of p.substeps_per_pass.
+// passes of p.substeps_per_pass.  With ck (sorted method, FP64) the first pass also
+// writes the state each parcel starts with into ck (6 arrays of c->n, device order).
 template <class T>
-int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p, bool capturing);
+int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p, bool capturing,
+               T *ck = nullptr);
 // The substep length the reference uses: Time stores dt as `scalar` (:272-275) and the
 // kernel divides by nSteps (:1261).
 inline double substep_dt(int prec, double dt_fluid, int nsub) {

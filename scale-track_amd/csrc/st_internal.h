@@ -66,6 +66,8 @@ struct PassArgs {
     Geo g;
     Phys<T> ph;
     double *part;    // per-workgroup statistics slots, or nullptr (no statistics this pass)
+    T *ck;           // checkpoint of the state each parcel starts the pass with (6 arrays of np,
+                     // device order), or nullptr; sorted method, FP64 only
     int32_t nbricks;
     int32_t nb[3];   // bricks per direction
     int32_t bd[3];   // cells per brick per direction

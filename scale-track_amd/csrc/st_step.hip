@@ -336,6 +336,18 @@ __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
     if (a.part) flush_stats(box, 0u, a.part + (size_t)blockIdx.x * kStatSlot);
 }
 
+// The adjoint's checkpoint of a parcel (the state it starts a segment with), written
+// from the registers the pass loaded it into instead of by a separate copy kernel.
+template <class T>
+__device__ __forceinline__ void save_checkpoint(T *ck, uint32_t n, uint32_t p, const ParcelIn<T> &in) {
+    ck[p] = in.x;
+    ck[n + p] = in.y;
+    ck[2 * n + p] = in.z;
+    ck[3 * n + p] = in.u;
+    ck[4 * n + p] = in.v;
+    ck[5 * n + p] = in.w;
+}
+
 // Tile cells handled per thread per round of the staging and flush loops.
 constexpr int kStage = 4;
 
@@ -357,7 +369,7 @@ struct TileShape {
 
 // One brick of a sorted cloud: stage its velocities in LDS, step its parcels, add its
 // source sums to HBM once per cell.  Every thread of the workgroup must call it.
-template <class T, bool PERIODIC, bool RHO>
+template <class T, bool PERIODIC, bool RHO, bool CKPT = false>
 __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t brick, int32_t nsteps,
                                               double *lds, double *part) {
     const int32_t bi = brick % a.nb[0];
@@ -462,6 +474,7 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             const bool stats = part != nullptr;
             auto one = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
                 const uint32_t q = ahead(p);
+                if (CKPT) save_checkpoint(a.ck, a.np, (uint32_t)p, cur);
                 step_parcel_one<T, PERIODIC, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, env, box, stats,
                                                   [&] { load_parcel<T, RHO>(a.s, q, a.ph.rho, fill); });
                 p += bs;
@@ -475,6 +488,7 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
         } else {
             auto many = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
                 const uint32_t q = ahead(p);
+                if (CKPT) save_checkpoint(a.ck, a.np, (uint32_t)p, cur);
                 step_parcel<T, PERIODIC, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, nsteps, env, box,
                                               [&] { load_parcel<T, RHO>(a.s, q, a.ph.rho, fill); });
                 p += bs;
@@ -540,14 +554,14 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     if (part) flush_stats(box, env.strays, part + (size_t)brick * kStatSlot);
 }
 
-template <class T, bool PERIODIC, bool RHO>
+template <class T, bool PERIODIC, bool RHO, bool CKPT = false>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4))) k_pass_sorted(PassArgs<T> a) {
     extern __shared__ double lds[];
     // Bricks in dispatch order: workgroups go round-robin over the XCDs, so the bricks
     // in flight at any time form one contiguous window of the parcel arrays (measured
     // faster on MI355X than giving each XCD its own contiguous range of bricks, which
     // spreads the concurrent streams over eight windows).
-    process_brick<T, PERIODIC, RHO>(a, blockIdx.x, a.nsteps, lds, a.part);
+    process_brick<T, PERIODIC, RHO, CKPT>(a, blockIdx.x, a.nsteps, lds, a.part);
 }
 
 // Many clouds in one persistent launch: each workgroup takes (cloud, brick) work items
@@ -648,6 +662,19 @@ template <class T>
 hipError_t launch_pass_sorted(const PassArgs<T> &a, int periodic, int block, hipStream_t s) {
     const size_t lds = (size_t)a.tile_cells * 3 * (sizeof(double) + sizeof(T));
     const bool rho = a.s.rho != nullptr;
+    if constexpr (sizeof(T) == 8) {
+        if (a.ck) {
+            if (periodic && rho)
+                k_pass_sorted<T, true, true, true><<<a.nbricks, block, lds, s>>>(a);
+            else if (periodic)
+                k_pass_sorted<T, true, false, true><<<a.nbricks, block, lds, s>>>(a);
+            else if (rho)
+                k_pass_sorted<T, false, true, true><<<a.nbricks, block, lds, s>>>(a);
+            else
+                k_pass_sorted<T, false, false, true><<<a.nbricks, block, lds, s>>>(a);
+            return hipGetLastError();
+        }
+    }
     if (periodic && rho)
         k_pass_sorted<T, true, true><<<a.nbricks, block, lds, s>>>(a);
     else if (periodic)
