@@ -350,6 +350,8 @@ __device__ __forceinline__ void save_checkpoint(T *ck, uint32_t n, uint32_t p, c
 
 // Tile cells handled per thread per round of the staging and flush loops.
 constexpr int kStage = 4;
+// Parcels per own cell from which a brick's source sums start from the old values.
+constexpr int32_t kSeedDensity = 6;
 
 // Index -> (i, j, k) of a tile of extent wx * wy * (any) cells, x fastest.  The
 // quotients come from a float reciprocal: for t below 2^22 the rounding error of
@@ -411,17 +413,27 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     ParcelIn<T> A, B;
     if (p1 > p0) load_parcel<T, RHO>(a.s, min((uint32_t)p, last), a.ph.rho, A);
 
+    // Owner flush of a dense brick (at least kSeedDensity parcels per own cell, so that
+    // nearly every own cell takes a deposit: with parcels spread at random a fraction
+    // of about exp(-kSeedDensity) does not): the own cells' sums start from the values in
+    // `src`, read with the velocities, and the flush stores them without a second
+    // memory round trip.  Only this workgroup touches its own cells of `src` during
+    // the launch, so the value read here is the one the flush overwrites.  A sparse
+    // brick keeps zero-started sums and updates only the cells that took a deposit.
+    const bool seeded = a.flush == 0 && p1 - p0 >= kSeedDensity * ex * ey * ez;
+
     // Stage the tile's velocities and clear its sums, kStage cells per thread at a
     // time with every load issued before the first LDS write (one memory round trip per
     // kStage*blockDim cells).  Halo cells outside the mesh are never looked up (every
     // cell index is folded or clamped into the mesh): they read a clamped cell and
-    // store zeros, so the loads need no condition.  Two lanes may write the same cell
-    // (the tile's last) with the same values: no barrier separates LDS writes of equal
-    // data, and the barrier below orders them before any read.
+    // store zeros, so the velocity loads need no condition.  Two lanes may write the
+    // same cell (the tile's last) with the same values: no barrier separates LDS writes
+    // of equal data, and the barrier below orders them before any read.
     const TileShape ts(wx, wy);
     for (int32_t t0 = 0; t0 < ncell; t0 += kStage * bs) {
         T u[kStage][3];
-        bool in[kStage];
+        double o[kStage][3];
+        bool in[kStage], own[kStage];
 #pragma unroll
         for (int k = 0; k < kStage; ++k) {
             const int32_t t = min(t0 + k * bs + (int32_t)threadIdx.x, ncell - 1);
@@ -430,12 +442,21 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             const int32_t gi = env.h0 + li, gj = env.h1 + lj, gk = env.h2 + lk;
             in[k] = (uint32_t)gi < (uint32_t)a.g.n[0] && (uint32_t)gj < (uint32_t)a.g.n[1] &&
                     (uint32_t)gk < (uint32_t)a.g.n[2];
+            own[k] = seeded && (uint32_t)(li - 1) < (uint32_t)ex && (uint32_t)(lj - 1) < (uint32_t)ey &&
+                     (uint32_t)(lk - 1) < (uint32_t)ez;
             const uint32_t c = (uint32_t)clampi(gk, a.g.n[2]) * a.g.nxny +
                                (uint32_t)clampi(gj, a.g.n[1]) * (uint32_t)a.g.n[0] + (uint32_t)clampi(gi, a.g.n[0]);
             const T *q = a.U + 3u * c;
             u[k][0] = q[0];
             u[k][1] = q[1];
             u[k][2] = q[2];
+            o[k][0] = o[k][1] = o[k][2] = 0.0;
+            if (own[k]) {
+                const double *r = a.src + 3u * c;
+                o[k][0] = r[0];
+                o[k][1] = r[1];
+                o[k][2] = r[2];
+            }
         }
         // Lanes past the tile rewrite its last cell with the same values, so every load
         // is consumed on every path and none is left in flight around the loop.
@@ -445,9 +466,9 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             Ut[3 * t] = in[k] ? u[k][0] : (T)0;
             Ut[3 * t + 1] = in[k] ? u[k][1] : (T)0;
             Ut[3 * t + 2] = in[k] ? u[k][2] : (T)0;
-            St[3 * t] = 0.0;
-            St[3 * t + 1] = 0.0;
-            St[3 * t + 2] = 0.0;
+            St[3 * t] = o[k][0];
+            St[3 * t + 1] = o[k][1];
+            St[3 * t + 2] = o[k][2];
         }
     }
     __syncthreads();
@@ -509,11 +530,12 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     // (deposits from other workgroups' halos and strays go to `stray`), so a plain
     // read-modify-write is race-free there; the halo's sums go to `stray` with atomics.
     // A cell with a nonzero sum lies inside the mesh (deposits go to folded or clamped
-    // cells only).
+    // cells only).  A seeded brick's own sums already hold the old values: they are
+    // stored without a read.
     for (int32_t t0 = 0; t0 < ncell; t0 += kStage * bs) {
         double v[kStage][3], old[kStage][3];
         uint32_t c[kStage];
-        bool nz[kStage], own[kStage];
+        bool live[kStage], nz[kStage], own[kStage];
 #pragma unroll
         for (int k = 0; k < kStage; ++k) {
             const int32_t t = t0 + k * bs + (int32_t)threadIdx.x;
@@ -521,14 +543,16 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             v[k][0] = St[3 * tc];
             v[k][1] = St[3 * tc + 1];
             v[k][2] = St[3 * tc + 2];
-            nz[k] = t < ncell && (v[k][0] != 0.0 || v[k][1] != 0.0 || v[k][2] != 0.0);
+            live[k] = t < ncell;
+            nz[k] = live[k] && (v[k][0] != 0.0 || v[k][1] != 0.0 || v[k][2] != 0.0);
             int32_t li, lj, lk;
             ts.split(tc, li, lj, lk);
             c[k] = (uint32_t)(env.h2 + lk) * a.g.nxny + (uint32_t)(env.h1 + lj) * (uint32_t)a.g.n[0] +
                    (uint32_t)(env.h0 + li);
             own[k] = a.flush == 0 && (uint32_t)(li - 1) < (uint32_t)ex && (uint32_t)(lj - 1) < (uint32_t)ey &&
                      (uint32_t)(lk - 1) < (uint32_t)ez;
-            if (nz[k] && own[k]) {
+            old[k][0] = old[k][1] = old[k][2] = 0.0;
+            if (!seeded && nz[k] && own[k]) {
                 const double *q = a.src + 3u * c[k];
                 old[k][0] = q[0];
                 old[k][1] = q[1];
@@ -537,12 +561,12 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
         }
 #pragma unroll
         for (int k = 0; k < kStage; ++k) {
-            if (!nz[k]) continue;
+            if (!(seeded && own[k] ? live[k] : nz[k])) continue;
             if (own[k]) {
                 double *q = a.src + 3u * c[k];
-                q[0] = old[k][0] + v[k][0];
-                q[1] = old[k][1] + v[k][1];
-                q[2] = old[k][2] + v[k][2];
+                q[0] = seeded ? v[k][0] : old[k][0] + v[k][0];
+                q[1] = seeded ? v[k][1] : old[k][1] + v[k][1];
+                q[2] = seeded ? v[k][2] : old[k][2] + v[k][2];
             } else {
                 double *q = env.out + 3u * c[k];
                 atomic_add_global(q, v[k][0]);
