@@ -201,3 +201,55 @@ def test_fp64_quotients_tiny_zero_and_near_midpoints(ctx, method, fused):
     assert np.abs(src - ref_src).max() <= 1e-12 * max(np.abs(ref_src).max(), 1e-300)
     chunk.close()
     field.close()
+
+
+def layered_case(dtype, boundary, seed=47):
+    """24x16x8 cells in 8x8x4 bricks (three brick columns along x): ~10 parcels per
+    cell in the first column, ~4 in the second, ~0.2 in the third."""
+    n = (24, 16, 8)
+    o, e = (0.0, 0.0, 0.0), (1.0, 0.75, 0.4)
+    third = (e[0] - o[0]) / 3.0
+    parts = []
+    for k, count in enumerate((10240, 4096, 400)):
+        lo = (o[0] + k * third, o[1], o[2])
+        hi = (o[0] + (k + 1) * third, e[1], e[2])
+        parts.append(inputs.positions(seed + k, count, lo, hi, dtype=dtype))
+    x, y, z = [np.concatenate([p[a] for p in parts]) for a in range(3)]
+    npc = x.size
+    d = inputs.lognormal_diameters(seed, npc, dtype=dtype)
+    zero = np.zeros(npc, dtype=dtype)
+    U = inputs.taylor_green(n, o, e, phase=(0.3, 0.9, 0.0), dtype=dtype)
+    return inputs.Case(n, o, e, U, d, x, y, z, zero, zero.copy(), zero.copy(), boundary=boundary,
+                       dt_fluid=1e-2, nsub=5)
+
+
+@pytest.mark.parametrize("per", [1, 0])
+@pytest.mark.parametrize("precision", [stk.FP64, stk.FP32])
+@pytest.mark.parametrize("boundary", [stk.WALL, stk.PERIODIC])
+def test_owner_flush_dense_and_sparse_bricks_accumulate(ctx, per, precision, boundary):
+    """The owner flush of a brick with at least 6 parcels per own cell starts its sums
+    from the source already in HBM; sparser bricks add their sums to it.  Bricks of
+    both kinds share the launch, and the source is never zeroed between calls, so
+    every call accumulates on what the earlier ones left: the result must be the sum
+    of the oracle's per-call sources."""
+    dt = np.float64 if precision == stk.FP64 else np.float32
+    case = layered_case(dt, boundary)
+    cells_per_brick = 8 * 8 * 4
+    counts = np.bincount(np.minimum((case.x.astype(np.float64) * 3.0).astype(int), 2), minlength=3) / 4
+    assert counts[0] >= 6 * cells_per_brick and counts[1] < 6 * cells_per_brick  # both kinds present
+    chunk, field = cloud(ctx, case, precision)
+    p = stk.params(boundary=boundary, method=stk.SORTED, substeps_per_pass=per, brick=(8, 8, 4))
+    state = case.state()
+    total = None
+    oprec = st_oracle.FP64 if precision == stk.FP64 else st_oracle.FP32
+    for call in range(3):
+        stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p)
+        ref_src, _, state = st_oracle.evolve_case(case, state=state, precision=oprec, substeps_per_pass=per)
+        total = ref_src if total is None else total + ref_src
+        src = field.download_source()
+        scale = max(np.abs(total).max(), 1e-300)
+        assert np.abs(src - total).max() <= 1e-12 * scale, f"call {call}"
+    for name, a, b in zip("dxyzuvw", chunk.download(), state):
+        assert np.array_equal(a, b), name
+    chunk.close()
+    field.close()
