@@ -241,7 +241,11 @@ int range_check(st_cloud *c, st_field *f) {
     unsigned int bad = 0;
     ST_HIP(hipMemcpyAsync(&bad, c->bad, 4, hipMemcpyDeviceToHost, s));
     ST_HIP(hipStreamSynchronize(s));
-    if (bad) return fail(ST_E_RANGE, "%u parcels lie outside the mesh [origin, ending)", bad);
+    if (bad)
+        return fail(ST_E_RANGE,
+                    "%u parcels lie outside the mesh [origin, ending): their cell index "
+                    "floor((x - origin) / delta) is outside [0, N) on some axis",
+                    bad);
     c->fresh = false;
     return ST_OK;
 }

@@ -58,9 +58,64 @@ def positions(seed, n, origin, ending, dtype=np.float64, stream=10):
         if bad.any():
             x[bad] = np.nextafter(hi, np.asarray(-np.inf, dtype=dtype))
         lo = np.asarray(origin[a], dtype=dtype)
+        if float(lo) < origin[a]:  # origin rounded down to dtype
+            lo = np.nextafter(lo, np.asarray(np.inf, dtype=dtype))
         x = np.maximum(x, lo)
         out.append(x)
     return out
+
+
+def _ordered(x, it):
+    """IEEE values of one width mapped to integers in the same order."""
+    i = x.view(it)
+    return np.where(i >= 0, i, np.iinfo(it).min - i)
+
+
+def _from_ordered(k, it, dtype):
+    return np.where(k >= 0, k, np.iinfo(it).min - k).astype(it).view(dtype)
+
+
+def into_cells(x, origin, ending, ncell):
+    """`x` with every coordinate outside the cells [0, ncell) of the library's locate,
+    floor(((double)x - origin) * rdelta) with rdelta = 1 / ((ending - origin) / ncell)
+    in double precision, replaced by the nearest value of its type inside them.  A
+    coordinate within rounding of `ending` (or, after rounding to float, just below
+    `origin`) can otherwise compute to cell ncell (or -1) although it lies inside
+    [origin, ending); the library refuses such a parcel on upload.  The nearest value
+    is found by bisection over the values of the type (the cell is monotone in x)."""
+    x = np.array(x, copy=True)
+    dt = x.dtype
+    it = np.int32 if dt == np.float32 else np.int64
+    rd = 1.0 / ((ending - origin) / ncell)
+    delta = (ending - origin) / ncell
+
+    def cell(v):
+        return np.floor((v.astype(np.float64) - origin) * rd)
+
+    for side in (1, -1):
+        bad = np.flatnonzero(cell(x) >= ncell) if side > 0 else np.flatnonzero(cell(x) < 0)
+        if bad.size == 0:
+            continue
+        centre = np.asarray(origin + (ncell - 0.5 if side > 0 else 0.5) * delta, dtype=dt)
+        good = np.full(bad.size, _ordered(centre.reshape(1), it)[0])
+        edge = _ordered(x[bad], it)
+        lo, hi = (good, edge) if side > 0 else (edge, good)  # lo < hi in value
+        while np.any(hi - lo > 1):
+            mid = lo + (hi - lo) // 2
+            out = cell(_from_ordered(mid, it, dt))
+            past = out >= ncell if side > 0 else out < 0
+            # keep the in-cell end on the good side, the outside end on the bad side
+            if side > 0:
+                hi = np.where(past, mid, hi)
+                lo = np.where(past, lo, mid)
+            else:
+                lo = np.where(past, mid, lo)
+                hi = np.where(past, hi, mid)
+        x[bad] = _from_ordered(lo if side > 0 else hi, it, dt)
+    c = cell(x)
+    if np.any((c < 0) | (c >= ncell)):
+        raise ValueError("coordinates could not be moved into the mesh's cells")
+    return x
 
 
 def lognormal_diameters(seed, n, median=30e-6, sigma=0.5, dtype=np.float64, stream=20):
@@ -147,6 +202,7 @@ class Case:
                  rho_d=1000.0, rho_c=1000.0, dt_fluid=1e-3, nsub=10, boundary=0, name=""):
         self.n, self.origin, self.ending = tuple(n), tuple(origin), tuple(ending)
         self.U = U
+        x, y, z = [into_cells(c, self.origin[a], self.ending[a], self.n[a]) for a, c in enumerate((x, y, z))]
         self.d, self.x, self.y, self.z, self.u, self.v, self.w = d, x, y, z, u, v, w
         self.rho = rho
         self.mu_c, self.rho_d, self.rho_c = mu_c, rho_d, rho_c

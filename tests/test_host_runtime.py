@@ -146,3 +146,51 @@ def test_vtu_round_trip(tmp_path):
     np.testing.assert_array_equal(a["types"], np.full(n, vtk.VTK_VERTEX))
     text = open(coll.path).read()
     assert 'timestep="0.01235"' in text and "dataVTK/particleFields_0.01235.vtu" in text
+
+
+def _cell(x, o, e, n):
+    return np.floor((np.asarray(x, dtype=np.float64) - o) * (1.0 / ((e - o) / n)))
+
+
+def test_into_cells_moves_only_coordinates_outside_the_cells():
+    """A coordinate inside [origin, ending) can compute to cell -1 or N through the
+    rounding of (x - origin) * rdelta, or of origin itself to float.  into_cells moves
+    exactly those, to the nearest value inside the cells, and leaves every other
+    coordinate alone."""
+    from scaletrack import inputs
+
+    # upper face, double: the largest double below ending computes to cell N
+    o, e, n = -0.9535514630027344, -0.023002589931070516, 17
+    top = np.nextafter(e, -np.inf)
+    assert top < e and _cell(top, o, e, n) == n
+    x = np.array([top, o, 0.5 * (o + e)])
+    y = inputs.into_cells(x, o, e, n)
+    assert np.all((_cell(y, o, e, n) >= 0) & (_cell(y, o, e, n) < n))
+    assert y[0] < top and _cell(np.nextafter(y[0], np.inf), o, e, n) == n  # the nearest value inside
+    np.testing.assert_array_equal(y[1:], x[1:])
+
+    # lower face, float: origin rounds below itself in float32 and computes to cell -1
+    o, e, n = -1.1443731422599845, 1.407537619706154, 26
+    low = np.float32(o)
+    assert float(low) < o and _cell(low, o, e, n) == -1
+    y = inputs.into_cells(np.array([low, np.float32(0.0)], dtype=np.float32), o, e, n)
+    assert y.dtype == np.float32 and _cell(y[0], o, e, n) == 0 and y[0] == np.nextafter(low, np.float32(1))
+    assert y[1] == 0.0
+
+
+def test_positions_and_cases_lie_in_the_cells():
+    """positions() keeps float coordinates at or above origin; a Case moves the rest of
+    its coordinates into the cells, so every synthetic cloud passes the upload check."""
+    from scaletrack import inputs
+
+    o, e, n = (-1.1443731422599845, -0.9535514630027344, 0.0), (1.407537619706154, -0.023002589931070516, 1.0), (26, 17, 4)
+    for dt in (np.float32, np.float64):
+        xs = inputs.positions(5, 20000, o, e, dtype=dt)
+        for a in range(3):
+            assert np.all(xs[a].astype(np.float64) >= o[a]) and np.all(xs[a].astype(np.float64) < e[a])
+        xs[1][0] = np.nextafter(e[1], -np.inf)  # computes to cell N in double
+        z = np.zeros(20000, dtype=dt)
+        case = inputs.Case(n, o, e, np.zeros((26 * 17 * 4, 3), dtype=dt), z + 1e-5, *xs, z, z, z)
+        for a, arr in enumerate((case.x, case.y, case.z)):
+            c = _cell(arr, o[a], e[a], n[a])
+            assert np.all((c >= 0) & (c < n[a])), a
