@@ -47,7 +47,8 @@ def oracle_dual(case, direction, cd=1.0, xw=None, vw=None, T=None):
     tsrc = np.zeros((case.ncell, 3))
     ts = np.zeros((6, case.np))
     src, _ = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
-                              boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d, rho_c=case.rho_c,
+                              boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d,
+                              rho_d_by_rho_c=case.rho_d_by_rho_c, rho_c=case.rho_c,
                               cd=cd, rho=case.rho, tan_d=direction[0], tan_rho=direction[1],
                               tan_cd=direction[2], tan_source=tsrc, tan_state=ts)
     val = 0.0
@@ -125,7 +126,8 @@ def test_adjoint_vs_frozen_event_fd(ctx, boundary):
     ev = np.zeros((case.np, case.nsub), dtype=np.int64)
     st = case.state()
     src0, _ = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
-                               boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d, rho_c=case.rho_c,
+                               boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d,
+                               rho_d_by_rho_c=case.rho_d_by_rho_c, rho_c=case.rho_c,
                                rho=case.rho, events=ev, events_mode=1)
     assert abs(loss - np.sum((src0 - T) ** 2)) <= 1e-12 * loss
     h = 1e-6
@@ -140,7 +142,8 @@ def test_adjoint_vs_frozen_event_fd(ctx, boundary):
         rho = case.rho + sign * h * dr if which == "rho" else case.rho
         cd = 1.0 + sign * h if which == "cd" else 1.0
         s, _ = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
-                                boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d, rho_c=case.rho_c,
+                                boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d,
+                                rho_d_by_rho_c=case.rho_d_by_rho_c, rho_c=case.rho_c,
                                 cd=cd, rho=rho, events=ev, events_mode=2)
         return np.sum((s - T) ** 2)
 

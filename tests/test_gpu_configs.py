@@ -129,7 +129,8 @@ def test_adjoint_config_size(ctx):
     st = case.state()
     tsrc = np.zeros((case.ncell, 3))
     src, _ = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
-                              boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d, rho_c=case.rho_c,
+                              boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d,
+                              rho_d_by_rho_c=case.rho_d_by_rho_c, rho_c=case.rho_c,
                               rho=case.rho, tan_d=dd, tan_rho=dr, tan_cd=dc, tan_source=tsrc)
     assert abs(loss - np.sum((src - T) ** 2)) <= 1e-12 * loss
     dual = np.sum(2.0 * (src - T) * tsrc)
@@ -138,7 +139,8 @@ def test_adjoint_config_size(ctx):
     # central differences with the forward run's decisions frozen
     ev = np.zeros((case.np, case.nsub), dtype=np.int64)
     st_oracle.evolve(case.state(), case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
-                     boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d, rho_c=case.rho_c, rho=case.rho,
+                     boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d,
+                     rho_d_by_rho_c=case.rho_d_by_rho_c, rho_c=case.rho_c, rho=case.rho,
                      events=ev, events_mode=1)
     h = 1e-6
 
@@ -146,7 +148,8 @@ def test_adjoint_config_size(ctx):
         st = case.state()
         st[0] = case.d + sign * h * dd
         s, _ = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
-                                boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d, rho_c=case.rho_c,
+                                boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d,
+                                rho_d_by_rho_c=case.rho_d_by_rho_c, rho_c=case.rho_c,
                                 cd=1.0 + sign * h * dc, rho=case.rho + sign * h * dr, events=ev, events_mode=2)
         return np.sum((s - T) ** 2)
 

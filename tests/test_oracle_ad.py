@@ -36,7 +36,7 @@ def run(case, d=None, rho=None, cd=1.0, events=None, mode=0, tan=None):
                   tan_source=np.zeros((case.ncell, 3)))
     src, _ = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid,
                               nsub=case.nsub, boundary=case.boundary, mu_c=case.mu_c,
-                              rho_d=case.rho_d, rho_c=case.rho_c, cd=cd,
+                              rho_d=case.rho_d, rho_d_by_rho_c=case.rho_d_by_rho_c, rho_c=case.rho_c, cd=cd,
                               rho=case.rho if rho is None else rho, events=events,
                               events_mode=mode, **kw)
     return src, st, kw.get("tan_source")
