@@ -187,3 +187,46 @@ def test_autotune_restores_state_and_caches(ctx, tmp_path):
         src = f.download_source()
         assert np.abs(src - ref_src).max() <= 1e-12 * max(np.abs(ref_src).max(), 1e-300)
     batch.close()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_single_cloud_graph_replays(ctx, seed):
+    """A random configuration captured in a graph and replayed three times, an eager
+    step between replays.  With resort_fraction 0 every eager step re-sorts the cloud,
+    in place, so the replays run on a brick order the capture never saw."""
+    rng = np.random.default_rng(11000 + seed)
+    precision = stk.FP32 if rng.random() < 0.5 else stk.FP64
+    dt = np.float32 if precision == stk.FP32 else np.float64
+    boundary = int(rng.integers(0, 2))
+    n = tuple(int(v) for v in rng.integers(2, 25, size=3))
+    o = tuple(float(v) for v in rng.uniform(-1.0, 1.0, size=3))
+    e = tuple(a + float(L) for a, L in zip(o, rng.uniform(0.2, 2.0, size=3)))
+    npc = int(rng.choice([1, 64, int(rng.integers(100, 20001))]))
+    x, y, z = inputs.positions(700 + seed, npc, o, e, dtype=dt)
+    d = inputs.lognormal_diameters(700 + seed, npc, median=float(10 ** rng.uniform(-5, -4)), dtype=dt)
+    u, v, w = [(rng.standard_normal(npc) * 0.3).astype(dt) for _ in range(3)]
+    U = inputs.fourier_field(n, o, e, modes=5, seed=800 + seed, amplitude=float(10 ** rng.uniform(-1, 0.5)), dtype=dt)
+    nsub = int(rng.integers(1, 9))
+    case = inputs.Case(n, o, e, U, d, x, y, z, u, v, w, dt_fluid=float(10 ** rng.uniform(-4, -2.5)), nsub=nsub,
+                       boundary=boundary)
+    per = int(rng.choice([0, 1, 2]))
+    p = stk.params(boundary=boundary, method=stk.SORTED if rng.random() < 0.7 else stk.PLAIN,
+                   substeps_per_pass=per, block=int(rng.choice([128, 256, 512])),
+                   resort_fraction=float(rng.choice([0.0, 0.08, 1e9])))
+    chunk, field = gpu_cloud(ctx, case, precision)
+    oprec = st_oracle.FP32 if precision == stk.FP32 else st_oracle.FP64
+    g = stk.Graph(ctx, chunk, field, case.dt_fluid, case.nsub, p, zero_source=True)
+    state = case.state()
+    for replay in range(3):
+        g.launch()
+        ref_src, _, state = st_oracle.evolve_case(case, state=state, precision=oprec, substeps_per_pass=per)
+        for name, a, b in zip("dxyzuvw", chunk.download(), state):
+            assert np.array_equal(a, b), (seed, replay, name)
+        src = field.download_source()
+        assert np.abs(src - ref_src).max() <= 1e-12 * max(np.abs(ref_src).max(), 1e-300), (seed, replay)
+        field.zero_source()
+        stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p)
+        _, _, state = st_oracle.evolve_case(case, state=state, precision=oprec, substeps_per_pass=per)
+    g.close()
+    chunk.close()
+    field.close()
