@@ -60,3 +60,49 @@ def test_more_than_one_gpu_is_refused():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU only" in r.stdout + r.stderr
+
+
+def _csv(path, header, rows):
+    import csv
+
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(header)
+        for r in rows:
+            w.writerow(r)
+    return str(path)
+
+
+def test_rocprof_summary_calibrates_and_sums_counter_instances(tmp_path, capsys):
+    """Synthetic rocprofv3 CSVs: the kernel's FETCH_SIZE comes in two instance rows per
+    dispatch (summed), the copy of 1 GiB reads half its bytes in FETCH_SIZE (factor 2)
+    and all of them in WRITE_SIZE (factor 1); the traffic is each kernel counter times
+    its factor, and the VALU issue rate is instructions / SIMDs / duration against one
+    wave64 instruction per 4 cycles."""
+    from scaletrack import rocprof_summary as rs
+
+    kern, copy = "st::k_pass_sorted<float, true, false, false>(x)", "st::k_copy(a, b, n)"
+    stats = _csv(tmp_path / "stats.csv", ["Name", "Calls", "TotalDurationNs", "AverageNs", "Percentage"],
+                 [[kern, 2, 400000, 200000, 80.0], [copy, 1, 100000, 100000, 20.0]])
+    head = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+    fetch = _csv(tmp_path / "fetch.csv", head,
+                 [[1, kern, "FETCH_SIZE", 100000.0], [1, kern, "FETCH_SIZE", 184289.0],
+                  [2, kern, "FETCH_SIZE", 150000.0], [2, kern, "FETCH_SIZE", 134289.0],
+                  [3, copy, "FETCH_SIZE", 524288.0]])
+    write = _csv(tmp_path / "write.csv", head,
+                 [[1, kern, "WRITE_SIZE", 463318.0], [2, kern, "WRITE_SIZE", 463318.0],
+                  [3, copy, "WRITE_SIZE", 1048576.0]])
+    pmc = _csv(tmp_path / "pmc.csv", head,
+               [[1, kern, "SQ_INSTS_VALU", 6.0e7], [1, kern, "SQ_WAVES", 3.0e4], [1, kern, "GRBM_GUI_ACTIVE", 1e6]])
+    rs.main(["--stats", stats, "--fetch", fetch, "--write", write, "--pmc", pmc, "--simds", "1000",
+             "--kernel", "k_pass_sorted", "--algorithmic-bytes", "1000000000", "--set", "precision=fp32"])
+    out = capsys.readouterr().out
+    summary = tmp_path / "summary.md"
+    summary.write_text(out)
+    fig = rs.read_figures(str(summary))
+    assert fig["fetch_factor"] == "2.0000" and fig["write_factor"] == "1.0000"
+    assert int(fig["traffic_bytes"]) == 284289 * 1024 * 2 + 463318 * 1024
+    assert fig["kernel_avg_ns"] == "200000" and fig["precision"] == "fp32"
+    assert fig["valu_per_wave"] == "2000"
+    # 6e7 / 1000 SIMDs / 200 us = 3e8 per s, against 2.4e9 / 4 = 6e8 per s
+    assert fig["valu_issue_fraction"] == "0.500"
