@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--brick", type=str, default="32,8,2")
     ap.add_argument("--block", type=int, default=512)
     ap.add_argument("--flush", choices=["owner", "atomic"], default="owner")
+    ap.add_argument("--passes-per-launch", type=int, default=1,
+                    help="passes of one persistent launch (each workgroup owning the same bricks)")
     ap.add_argument("--only", action="store_true",
                     help="time only the headline configuration (for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -89,6 +91,8 @@ def run_variant(stk, ctx, case, precision, p, steps, warmup, keep_state=False):
         stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
         if keep_state:
             srcs.append(field.download_source().copy())
+    sorted_method = p.method == stk.SORTED
+    resorts0 = chunk.resort_count() if sorted_method else 0
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -106,7 +110,8 @@ def run_variant(stk, ctx, case, precision, p, steps, warmup, keep_state=False):
         "updates_per_s": case.np * case.nsub * steps / elapsed,
         "kernel_ms_per_pass": float(np.mean(kern_ms)),
         "passes_per_step": launches,
-        "stray_fraction": chunk.stray_fraction() if p.method == stk.SORTED else None,
+        "stray_fraction": chunk.stray_fraction() if sorted_method else None,
+        "resorts_in_timed_steps": chunk.resort_count() - resorts0 if sorted_method else None,
     }
     if keep_state:
         out["srcs"] = srcs
@@ -377,6 +382,7 @@ def main():
     brick = [int(v) for v in args.brick.split(",")]
     p = stk.params(boundary=stk.PERIODIC, method=stk.SORTED if args.method == "sorted" else stk.PLAIN,
                    substeps_per_pass=args.substeps_per_pass, brick=brick, block=args.block,
+                   passes_per_launch=args.passes_per_launch,
                    flush=stk.FLUSH_OWNER if args.flush == "owner" else stk.FLUSH_ATOMIC)
     head = run_variant(stk, ctx, case, precision, p, args.steps, args.warmup)
     print(f"# headline: {json.dumps(head)}", file=sys.stderr, flush=True)
@@ -409,6 +415,7 @@ def main():
             "brick": brick,
             "block": args.block,
             "flush": args.flush,
+            "passes_per_launch": args.passes_per_launch,
         },
         "roofline": {
             "bound": "hbm",

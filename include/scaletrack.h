@@ -16,6 +16,12 @@
  * host data synchronise that stream.  A handle must not be used from two threads at
  * once.  Cell fields are AoS xyz per cell, cells numbered x fastest (the reference's
  * VectorField, asyncParallelTracking.jl:186-205, and OpenFOAM's blockMesh order).
+ *
+ * Lifetimes: a graph, batch or adjoint keeps raw device pointers of the clouds and
+ * fields it was made from.  While one is alive those clouds and fields refuse (with
+ * ST_E_STATE, before any device work) to be destroyed, to be sorted by another brick or
+ * mesh, or to grow a buffer the graph or batch captured; destroy the graph, batch or
+ * adjoint first.  A batch refuses to be destroyed while a graph of it is alive.
  */
 #ifndef SCALETRACK_H
 #define SCALETRACK_H
@@ -26,7 +32,7 @@
 extern "C" {
 #endif
 
-#define ST_ABI_VERSION 1
+#define ST_ABI_VERSION 2
 
 typedef enum st_status {
     ST_OK = 0,
@@ -80,8 +86,13 @@ typedef struct st_params {
     double cd;                 /* drag-coefficient multiplier, default 1 (reference drag) */
     int32_t brick[3];          /* cells per brick (sorted method); default 32,8,2 */
     int32_t block;             /* threads per workgroup; default 512 */
-    double resort_fraction;    /* re-sort when the share of parcels outside their brick in
-                                  the last pass exceeds this; default 0.08 */
+    double resort_fraction;    /* sorted method: after each step the device re-sorts the
+                                  cloud when the share of parcels outside their brick in the
+                                  step's last pass exceeds this (decided on the device, also
+                                  inside captured graphs); default 0.08 */
+    int32_t passes_per_launch; /* sorted method: passes run by one persistent launch, each
+                                  workgroup owning the same bricks in every pass; 1 = one
+                                  launch per pass (default) */
 } st_params;
 
 /* ---- library ---------------------------------------------------------------- */
@@ -138,12 +149,17 @@ int st_evolve(st_ctx *ctx, st_cloud *cloud, st_field *field, double dt_fluid, in
 /* Share of parcels that were outside their sorted brick during the last sorted pass
  * (synchronises); -1 if unknown. */
 int st_cloud_stray_fraction(st_cloud *cloud, double *fraction);
+/* Re-sorts the device has decided on for this cloud so far (params->resort_fraction);
+ * synchronises.  No counterpart in the reference, whose Hilbert-ordered chunks keep
+ * their initial order (asyncParallelTracking.jl:559-638). */
+int st_cloud_resort_count(st_cloud *cloud, int64_t *count);
 /* Forces a sort of the cloud by brick of `field`'s mesh before the next step. */
 int st_cloud_sort(st_ctx *ctx, st_cloud *cloud, st_field *field, const st_params *params);
 
 /* ---- HIP graph of one fluid step -------------------------------------------- */
 /* Captures zeroing UTrans plus the nsub substeps as one hipGraph (sorted method: the
- * cloud is sorted first, outside the graph; no re-sort happens inside it). */
+ * cloud is sorted first, outside the graph; the device-decided re-sort behind the step
+ * is part of the graph). */
 int st_graph_capture(st_ctx *ctx, st_cloud *cloud, st_field *field, double dt_fluid,
                      int32_t nsub, const st_params *params, int32_t zero_source,
                      st_graph **out);

@@ -23,6 +23,7 @@ struct st_adjoint {
     uint32_t n = 0;
     bool forward_done = false;
     bool backward_done = false;
+    int64_t epoch = -1;  // the cloud's epoch right after the forward
     double *ck = nullptr;    // nseg * 6 * n
     double *xb = nullptr;    // 3n
     double *vb = nullptr;    // 3n, propagated by the backward sweep
@@ -41,6 +42,8 @@ namespace {
 int destroy(st_adjoint *a) {
     if (!a) return ST_OK;
     quiesce_ctx(a->ctx);
+    if (a->cloud) unpin(a->cloud);
+    if (a->field) unpin(a->field);
     for (double *q : {a->ck, a->xb, a->vb, a->vseed, a->fbar, a->mbar, a->G, a->UG, a->T, a->tmp, a->scal})
         if (q) (void)hipFree(q);
     st_ctx *ctx = a->ctx;
@@ -52,6 +55,15 @@ int destroy(st_adjoint *a) {
 int check(st_adjoint *a) {
     if (!a) return fail(ST_E_INVALID, "null adjoint handle");
     return check_ctx(a->ctx);
+}
+
+// The reverse sweep replays the forward from its checkpoints in the device order of
+// the forward: a cloud stepped, sorted or uploaded since then has another state.
+int check_unchanged(st_adjoint *a) {
+    if (a->forward_done && a->cloud->epoch != a->epoch)
+        return fail(ST_E_STATE,
+                    "the cloud was stepped, sorted or uploaded after st_adjoint_forward; run the forward again");
+    return ST_OK;
 }
 
 }  // namespace
@@ -73,6 +85,8 @@ int st_adjoint_create(st_ctx *ctx, st_cloud *c, st_field *f, int32_t nsub, int32
     retain_ctx(ctx);
     a->cloud = c;
     a->field = f;
+    pin(c);
+    pin(f);
     a->p = p;
     a->p.substeps_per_pass = checkpoint_every;
     a->nsub = nsub;
@@ -113,6 +127,8 @@ int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
     hipStream_t s = a->ctx->stream;
     if (p.method == ST_METHOD_SORTED && !sort_matches(c, f, p)) ST_TRY(sort_any(c, f, p));
     ST_TRY(st_field_zero_source(f));
+    StatsBuf sb;
+    ST_TRY(cloud_stats(c, stats_slots(c, p), sb));
     a->dt_sub = substep_dt(ST_FP64, dt_fluid, a->nsub);
     const uint32_t n = a->n;
     for (int32_t seg = 0; seg < a->nseg; ++seg) {
@@ -120,12 +136,12 @@ int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
         const int32_t ks = std::min(a->K, a->nsub - seg * a->K);
         if (p.method == ST_METHOD_SORTED) {
             // the segment's first pass writes the checkpoint from the state it loads
-            ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false, ck));
+            ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false, sb, false, ck));
         } else {
             st::Soa<double> cur = soa<double>(c, c->cur);
             const double *fields[6] = {cur.x, cur.y, cur.z, cur.u, cur.v, cur.w};
             ST_HIP(st::launch_save6(fields, ck, n, s));
-            ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false));
+            ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false, sb, false));
         }
     }
     ST_HIP(st::launch_fill_f64(a->xb, 0.0, 3 * (size_t)n, s));
@@ -133,6 +149,7 @@ int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
     ST_HIP(st::launch_fill_f64(a->G, 0.0, (size_t)f->ncell * 3, s));
     a->forward_done = true;
     a->backward_done = false;
+    a->epoch = c->epoch;
     return ST_OK;
 }
 
@@ -166,6 +183,7 @@ int st_adjoint_set_source_cotangent(st_adjoint *a, const double *g) {
 int st_adjoint_set_state_cotangent(st_adjoint *a, const double *xbar, const double *vbar) {
     ST_TRY(check(a));
     if (!a->forward_done) return fail(ST_E_STATE, "run st_adjoint_forward first");
+    ST_TRY(check_unchanged(a));
     hipStream_t s = a->ctx->stream;
     const uint32_t n = a->n;
     const int32_t *pid = a->cloud->pid[a->cloud->cur];
@@ -184,6 +202,7 @@ int st_adjoint_set_state_cotangent(st_adjoint *a, const double *xbar, const doub
 int st_adjoint_backward(st_adjoint *a) {
     ST_TRY(check(a));
     if (!a->forward_done) return fail(ST_E_STATE, "run st_adjoint_forward first");
+    ST_TRY(check_unchanged(a));
     st_cloud *c = a->cloud;
     st_field *f = a->field;
     hipStream_t s = a->ctx->stream;
@@ -226,6 +245,7 @@ int st_adjoint_gradients(st_adjoint *a, double *grad_d, double *grad_rho, double
                          double *grad_v0) {
     ST_TRY(check(a));
     if (!a->backward_done) return fail(ST_E_STATE, "run st_adjoint_forward and st_adjoint_backward first");
+    ST_TRY(check_unchanged(a));
     st_cloud *c = a->cloud;
     if (grad_rho && !c->has_rho) return fail(ST_E_INVALID, "grad_rho needs a cloud with per-parcel density");
     hipStream_t s = a->ctx->stream;

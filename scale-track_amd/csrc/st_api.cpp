@@ -143,6 +143,8 @@ int validate_params(const st_params &p) {
     if (p.block != 64 && p.block != 128 && p.block != 256 && p.block != 512 && p.block != 1024)
         return fail(ST_E_INVALID, "block must be 64, 128, 256, 512 or 1024 (got %d)", p.block);
     if (!(p.resort_fraction >= 0)) return fail(ST_E_INVALID, "resort_fraction must be >= 0");
+    if (p.passes_per_launch < 1 || p.passes_per_launch > 1024)
+        return fail(ST_E_INVALID, "passes_per_launch must be in [1, 1024] (got %d)", p.passes_per_launch);
     return ST_OK;
 }
 
@@ -150,9 +152,78 @@ int validate_params(const st_params &p) {
 // larger ones through one global counter per brick.
 constexpr int64_t kSortLdsBins = 8192;
 
+void pin(st_cloud *c) { ++c->pins; }
+void unpin(st_cloud *c) { --c->pins; }
+void pin(st_field *f) { ++f->pins; }
+void unpin(st_field *f) { --f->pins; }
+
+// Histogram rows of the chunked sort (one per workgroup of at least 4096 parcels, at
+// most 512), or 0 when the brick count takes the global-counter path.
+static int32_t sort_rows(int64_t n, int64_t nbr) {
+    if (nbr > kSortLdsBins) return 0;
+    return (int32_t)std::max<int64_t>(1, std::min<int64_t>(512, (n + 4095) / 4096));
+}
+
+// Device buffer `*p` of at least `need` elements of `es` bytes.  Growing one that exists
+// frees it, which a pinned cloud refuses (captured graphs hold its address).
+template <class U>
+static int grow(st_cloud *c, U **p, size_t &cap, size_t need, const char *what) {
+    if (*p && need <= cap) return ST_OK;
+    if (*p) {
+        if (c->pins > 0)
+            return fail(ST_E_STATE,
+                        "the cloud's %s would be reallocated, but %d graph(s), batch(es) or adjoint(s) "
+                        "hold its device buffers; destroy them first",
+                        what, c->pins);
+        ST_HIP(hipFree(*p));
+        *p = nullptr;
+        cap = 0;
+    }
+    ST_HIP(hipMalloc((void **)p, need * sizeof(U)));
+    cap = need;
+    return ST_OK;
+}
+
+// Queues the sort of the cloud by brick (nb bricks per direction, nbr in all, rows as
+// sort_rows gives) into the second buffer set and its copy back over the first, all
+// gated on gate[0] when gate is set (a gated sort that runs counts itself in count).
+// Every buffer must exist.
+template <class T>
+static int enqueue_sort(st_cloud *c, st_field *f, const st_params &p, const int32_t nb[3], int64_t nbr,
+                        int32_t rows, const unsigned int *gate, unsigned int *count) {
+    hipStream_t s = c->ctx->stream;
+    const uint32_t np = (uint32_t)c->n;
+    ST_HIP(launch_fill_u32(c->bad, 0u, 1, s, gate));
+    if (rows > 0) {
+        ST_HIP(launch_sort_chunked<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur), c->pid[1 - c->cur],
+                                      np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC, rows, c->keys, c->hist,
+                                      c->counts, c->brick_start, c->cursor, c->bad, gate, s));
+    } else {
+        ST_HIP(launch_fill_u32(c->counts, 0u, (size_t)nbr, s, gate));
+        ST_HIP(launch_brick_keys<T>(soa<T>(c, c->cur), np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC,
+                                    c->keys, c->counts, c->bad, gate, s));
+        ST_HIP(launch_scan(c->counts, (int32_t)nbr, c->brick_start, c->cursor, gate, s));
+        ST_HIP(launch_scatter<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur), c->pid[1 - c->cur],
+                                 c->keys, c->cursor, np, gate, s));
+    }
+    // The sorted order is copied back so the cloud's device pointers never change:
+    // graphs and batch tables captured earlier stay valid across re-sorts.
+    ST_HIP(launch_copy_back<T>(soa<T>(c, 1 - c->cur), c->pid[1 - c->cur], soa<T>(c, c->cur), c->pid[c->cur], np,
+                               gate, count, s));
+    return ST_OK;
+}
+
+static bool same_geometry(const st_cloud *c, const st_field *f, const st_params &p) {
+    if (c->sort_periodic != p.boundary) return false;
+    for (int a = 0; a < 3; ++a)
+        if (c->sort_n[a] != f->mesh.n[a] || c->sort_bd[a] != p.brick[a] || c->sort_o[a] != f->mesh.origin[a] ||
+            c->sort_e[a] != f->mesh.ending[a])
+            return false;
+    return true;
+}
+
 template <class T>
 int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
-    hipStream_t s = c->ctx->stream;
     int32_t nb[3];
     int64_t nbr = 1;
     for (int a = 0; a < 3; ++a) {
@@ -160,52 +231,23 @@ int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
         nbr *= nb[a];
     }
     if (nbr > (1 << 26)) return fail(ST_E_RANGE, "too many bricks (%lld)", (long long)nbr);
-    if (nbr > c->bins_cap) {
-        if (c->counts) ST_HIP(hipFree(c->counts));
-        if (c->cursor) ST_HIP(hipFree(c->cursor));
-        if (c->brick_start) ST_HIP(hipFree(c->brick_start));
-        c->counts = nullptr;
-        c->cursor = nullptr;
-        c->brick_start = nullptr;
-        ST_HIP(hipMalloc((void **)&c->counts, (size_t)nbr * 4));
-        ST_HIP(hipMalloc((void **)&c->cursor, (size_t)nbr * 4));
-        ST_HIP(hipMalloc((void **)&c->brick_start, (size_t)(nbr + 1) * 4));
-        c->bins_cap = (int32_t)nbr;
-    }
-    if (!c->keys) ST_HIP(hipMalloc((void **)&c->keys, (size_t)(c->n > 0 ? c->n : 1) * 4));
+    if (c->pins > 0 && c->nbricks > 0 && !same_geometry(c, f, p))
+        return fail(ST_E_STATE,
+                    "%d graph(s), batch(es) or adjoint(s) captured this cloud sorted by brick %dx%dx%d of its "
+                    "mesh; a sort by another brick or mesh would change what they step: destroy them first",
+                    c->pins, c->sort_bd[0], c->sort_bd[1], c->sort_bd[2]);
+    const int32_t rows = sort_rows(c->n, nbr);
+    const size_t n1 = (size_t)(c->n > 0 ? c->n : 1);
+    size_t bins = (size_t)c->bins_cap, bins1 = (size_t)c->bins_cap + 1, keys_cap = c->keys ? n1 : 0;
+    size_t bins_c = bins;
+    ST_TRY(grow(c, &c->counts, bins, (size_t)nbr, "sort counters"));
+    ST_TRY(grow(c, &c->cursor, bins_c, (size_t)nbr, "sort cursors"));
+    ST_TRY(grow(c, &c->brick_start, bins1, (size_t)nbr + 1, "brick ranges"));
+    c->bins_cap = (int32_t)std::max<int64_t>(c->bins_cap, nbr);
+    ST_TRY(grow(c, &c->keys, keys_cap, n1, "sort keys"));
+    if (rows > 0) ST_TRY(grow(c, &c->hist, c->hist_cap, (size_t)rows * (size_t)nbr, "sort histograms"));
     ST_TRY(alloc_set(c, 1 - c->cur));
-    const uint32_t np = (uint32_t)c->n;
-    ST_HIP(launch_fill_u32(c->bad, 0u, 1, s));
-    if (nbr <= kSortLdsBins) {
-        // chunked: one histogram row per workgroup of at least 4096 parcels, at most 512 rows
-        const int32_t rows = (int32_t)std::max<int64_t>(1, std::min<int64_t>(512, ((int64_t)np + 4095) / 4096));
-        const size_t need = (size_t)rows * (size_t)nbr;
-        if (need > c->hist_cap) {
-            if (c->hist) ST_HIP(hipFree(c->hist));
-            c->hist = nullptr;
-            c->hist_cap = 0;
-            ST_HIP(hipMalloc((void **)&c->hist, need * 4));
-            c->hist_cap = need;
-        }
-        ST_HIP(launch_sort_chunked<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur), c->pid[1 - c->cur],
-                                      np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC, rows, c->keys, c->hist,
-                                      c->counts, c->brick_start, c->cursor, c->bad, s));
-    } else {
-        ST_HIP(launch_fill_u32(c->counts, 0u, (size_t)nbr, s));
-        ST_HIP(launch_brick_keys<T>(soa<T>(c, c->cur), np, f->geo, nb, p.brick, p.boundary == ST_PERIODIC,
-                                        c->keys, c->counts, c->bad, s));
-        ST_HIP(launch_scan(c->counts, (int32_t)nbr, c->brick_start, c->cursor, s));
-        ST_HIP(launch_scatter<T>(soa<T>(c, c->cur), c->pid[c->cur], soa<T>(c, 1 - c->cur),
-                                     c->pid[1 - c->cur], c->keys, c->cursor, np, s));
-    }
-    // Copy the sorted order back so the cloud's device pointers never change: graphs
-    // and batch tables captured earlier stay valid across re-sorts.
-    const size_t bytes = (size_t)c->n * c->es;
-    for (int f = 0; f < kNF; ++f) {
-        if (f == 7 && !c->has_rho) continue;
-        ST_HIP(hipMemcpyAsync(c->buf[c->cur][f], c->buf[1 - c->cur][f], bytes, hipMemcpyDeviceToDevice, s));
-    }
-    ST_HIP(hipMemcpyAsync(c->pid[c->cur], c->pid[1 - c->cur], (size_t)c->n * 4, hipMemcpyDeviceToDevice, s));
+    ST_TRY(enqueue_sort<T>(c, f, p, nb, nbr, rows, nullptr, nullptr));
     c->permuted = true;
     c->sorted = true;
     for (int a = 0; a < 3; ++a) {
@@ -219,7 +261,16 @@ int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
     c->nbricks = (int32_t)nbr;
     c->strays_pending = false;
     c->last_stray_fraction = -1.0;
+    ++c->epoch;
     return ST_OK;
+}
+
+// The re-sort queued behind a step of a sorted cloud, run only if the step's statistics
+// reduction set gate[0] (its buffers exist: the cloud was sorted with this geometry).
+template <class T>
+static int queue_resort(st_cloud *c, st_field *f, const st_params &p, const unsigned int *gate) {
+    const int64_t nbr = c->nbricks;
+    return enqueue_sort<T>(c, f, p, c->nb, nbr, sort_rows(c->n, nbr), gate, c->resort + 1);
 }
 
 bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p) {
@@ -250,9 +301,39 @@ int range_check(st_cloud *c, st_field *f) {
     return ST_OK;
 }
 
+// Persistent workgroups of the multi-pass launch: the most that fill every CU equally
+// (a multiple of the CU count, at most the kernel's occupancy per CU) and split the
+// bricks evenly, so no workgroup and no CU owns more bricks than another; the most that
+// fit when no such count exists.
+template <class T>
+static int multi_grid(const st_ctx *ctx, const st_cloud *c, const st_params &p, int32_t tile_cells) {
+    const size_t lds = (size_t)tile_cells * 3 * (sizeof(double) + sizeof(T));
+    const int per_cu = pass_multi_blocks_per_cu<T>(p.boundary == ST_PERIODIC, c->has_rho, p.block, lds);
+    if (per_cu < 1) return 0;
+    for (int m = per_cu; m >= 1; --m) {
+        const int64_t g = (int64_t)m * ctx->cus;
+        if (g <= c->nbricks && c->nbricks % g == 0) return (int)g;
+    }
+    return (int)std::min<int64_t>((int64_t)per_cu * ctx->cus, std::max<int32_t>(1, c->nbricks));
+}
+
+int32_t stats_slots(const st_cloud *c, const st_params &p) {
+    return p.method == ST_METHOD_SORTED ? c->nbricks : (int32_t)((c->n + p.block - 1) / p.block);
+}
+
+int cloud_stats(st_cloud *c, int32_t nslots, StatsBuf &sb) {
+    ST_TRY(ensure_part(c, nslots));
+    sb.part = c->part;
+    sb.cap = c->part_cap;
+    sb.bbox = c->bbox;
+    sb.strays = c->strays;
+    sb.gate = c->resort;
+    return ST_OK;
+}
+
 template <class T>
 int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p,
-               bool capturing, T *ck) {
+               bool capturing, const StatsBuf &sb, bool device_resort, T *ck) {
     hipStream_t s = ctx->stream;
     PassArgs<T> a{};
     a.s = soa<T>(c, c->cur);
@@ -273,31 +354,56 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     a.np = (uint32_t)c->n;
     const int per = p.substeps_per_pass > 0 ? p.substeps_per_pass : nsub;
     const int periodic = p.boundary == ST_PERIODIC;
+    // passes of one persistent launch (sorted method, no checkpoint)
+    const int group = p.method == ST_METHOD_SORTED && !ck ? std::max(1, p.passes_per_launch) : 1;
+    const int grid = group > 1 ? multi_grid<T>(ctx, c, p, a.tile_cells) : 0;
+    if (group > 1 && grid < 1) return fail(ST_E_RANGE, "the multi-pass kernel does not fit: block %d", p.block);
 
     // Statistics (bounding box, strays) are taken in the last pass only.
-    const int32_t nslots = p.method == ST_METHOD_SORTED ? c->nbricks : (int32_t)((c->n + p.block - 1) / p.block);
-    ST_TRY(ensure_part(c, nslots));
+    const int32_t nslots = stats_slots(c, p);
+    if (nslots > sb.cap)
+        return fail(ST_E_RANGE, "the step needs %d statistics slots; %d are available", nslots, sb.cap);
     if (!capturing) ST_HIP(timer_start(ctx, s));
-    int launches = 0;
-    for (int done = 0; done < nsub; done += per) {
+    int passes = 0;
+    for (int done = 0; done < nsub;) {
         a.nsteps = per < nsub - done ? per : nsub - done;
-        a.part = done + a.nsteps >= nsub ? c->part : nullptr;
+        // equal passes that follow are run by the same launch
+        int k = 1;
+        while (k < group && done + (k + 1) * a.nsteps <= nsub &&
+               (per < nsub - done - k * a.nsteps ? per : nsub - done - k * a.nsteps) == a.nsteps)
+            ++k;
+        const int next = done + k * a.nsteps;
+        a.part = next >= nsub ? sb.part : nullptr;
         a.ck = done == 0 ? ck : nullptr;
-        if (p.method == ST_METHOD_SORTED)
+        if (k > 1)
+            ST_HIP(launch_pass_multi<T>(a, k, periodic, p.block, grid, s));
+        else if (p.method == ST_METHOD_SORTED)
             ST_HIP(launch_pass_sorted<T>(a, periodic, p.block, s));
         else
             ST_HIP(launch_pass_plain<T>(a, periodic, p.block, s));
-        ++launches;
+        passes += k;
+        done = next;
     }
-    if (!capturing) ST_HIP(timer_stop(ctx, s, launches));
-    ST_HIP(launch_reduce_stats(StatsTarget{c->part, nslots, c->bbox, c->strays}, s));
+    // the recorded time covers every pass; per-pass figures divide by the pass count
+    if (!capturing) ST_HIP(timer_stop(ctx, s, passes));
+    const bool resort = device_resort && p.method == ST_METHOD_SORTED && c->sorted && c->n > 0;
+    StatsTarget t{sb.part, nslots, sb.bbox, sb.strays, nullptr, 0u};
+    if (resort) {
+        // share > resort_fraction  <=>  strays > floor(resort_fraction * n)
+        const double th = std::floor(p.resort_fraction * (double)c->n);
+        t.resort = sb.gate;
+        t.threshold = th >= 4294967295.0 ? 0xffffffffu : (uint32_t)th;
+    }
+    ST_HIP(launch_reduce_stats(t, s));
     if (p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER) f->stray_dirty = true;
     if (p.method == ST_METHOD_SORTED && !capturing) {
-        ST_HIP(hipMemcpyAsync(c->strays_host, c->strays, 4, hipMemcpyDeviceToHost, s));
+        ST_HIP(hipMemcpyAsync(c->strays_host, sb.strays, 4, hipMemcpyDeviceToHost, s));
         ST_HIP(hipEventRecord(c->strays_ev, s));
         c->strays_pending = true;
         c->strays_den = c->n;
     }
+    if (resort) ST_TRY(queue_resort<T>(c, f, p, sb.gate));
+    ++c->epoch;
     return ST_OK;
 }
 
@@ -320,14 +426,10 @@ int prepare(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int nsub, co
     return ST_OK;
 }
 
+// A host-side sort when the cloud is not sorted for this mesh and brick; re-sorts of a
+// sorted cloud are decided on the device (run_passes).
 int maybe_sort(st_cloud *c, st_field *f, const st_params &p) {
-    bool need = !sort_matches(c, f, p);
-    if (!need && c->strays_pending && hipEventQuery(c->strays_ev) == hipSuccess) {
-        c->strays_pending = false;
-        c->last_stray_fraction = c->strays_den > 0 ? (double)*c->strays_host / (double)c->strays_den : 0.0;
-        if (c->last_stray_fraction > p.resort_fraction) need = true;
-    }
-    if (!need) return ST_OK;
+    if (sort_matches(c, f, p)) return ST_OK;
     return c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p);
 }
 
@@ -338,6 +440,11 @@ int sort_any(st_cloud *c, st_field *f, const st_params &p) {
 
 int ensure_part(st_cloud *c, int32_t nslots) {
     if (nslots <= c->part_cap) return ST_OK;
+    if (c->part && c->pins > 0)
+        return fail(ST_E_STATE,
+                    "the cloud's statistics slots would be reallocated (%d -> %d), but %d graph(s), batch(es) "
+                    "or adjoint(s) hold them; destroy them first",
+                    c->part_cap, nslots, c->pins);
     if (c->part) ST_HIP(hipFree(c->part));
     c->part = nullptr;
     c->part_cap = 0;
@@ -376,8 +483,10 @@ Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd
 }
 template Phys<float> make_phys<float>(const st_cloud *, const st_field *, double, double);
 template Phys<double> make_phys<double>(const st_cloud *, const st_field *, double, double);
-template int run_passes<float>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool, float *);
-template int run_passes<double>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool, double *);
+template int run_passes<float>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool,
+                               const StatsBuf &, bool, float *);
+template int run_passes<double>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool,
+                                const StatsBuf &, bool, double *);
 
 }  // namespace host
 }  // namespace st
@@ -402,6 +511,7 @@ void st_params_default(st_params *p) {
     p->brick[2] = 2;
     p->block = 512;
     p->resort_fraction = 0.08;
+    p->passes_per_launch = 1;
 }
 
 int st_device_count(int32_t *count) {
@@ -435,6 +545,9 @@ int st_ctx_create(int32_t device, st_ctx **out) {
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess &&
         lds > 0)
         c->lds_per_cu = (size_t)lds;
+    if (e == hipSuccess && hipDeviceGetAttribute(&lds, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        lds > 0)
+        c->cus = lds;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int k = 0; k < kTimerRing && e == hipSuccess; ++k) {
         e = hipEventCreate(&c->tev[k][0]);
@@ -496,6 +609,8 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
     if (r == ST_OK) e = hipMalloc((void **)&c->bbox, 6 * sizeof(double));
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->strays, 4);
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->bad, 4);
+    if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->resort, 4 * sizeof(unsigned int));
+    if (r == ST_OK && e == hipSuccess) e = st::launch_fill_u32(c->resort, 0u, 4, ctx->stream);
     if (r == ST_OK && e == hipSuccess) e = hipHostMalloc((void **)&c->strays_host, 4, hipHostMallocDefault);
     if (r == ST_OK && e == hipSuccess) e = hipEventCreateWithFlags(&c->strays_ev, hipEventDisableTiming);
     if (r == ST_OK && e == hipSuccess) e = st::launch_iota(c->pid[0], (uint32_t)n, ctx->stream);
@@ -510,6 +625,9 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
 
 int st_cloud_destroy(st_cloud *c) {
     if (!c) return ST_OK;
+    if (c->pins > 0)
+        return fail(ST_E_STATE, "cloud is used by %d live graph(s), batch(es) or adjoint(s); destroy them first",
+                    c->pins);
     quiesce_ctx(c->ctx);
     for (int w = 0; w < 2; ++w) {
         for (int f = 0; f < kNF; ++f)
@@ -525,6 +643,7 @@ int st_cloud_destroy(st_cloud *c) {
     if (c->part) hipFree(c->part);
     if (c->strays) hipFree(c->strays);
     if (c->bad) hipFree(c->bad);
+    if (c->resort) hipFree(c->resort);
     if (c->strays_host) hipHostFree(c->strays_host);
     if (c->strays_ev) hipEventDestroy(c->strays_ev);
     st_ctx *ctx = c->ctx;
@@ -561,6 +680,7 @@ int st_cloud_upload(st_cloud *c, const void *d, const void *x, const void *y, co
     c->permuted = false;
     c->sorted = false;
     c->strays_pending = false;
+    ++c->epoch;
     return ST_OK;
 }
 
@@ -633,6 +753,9 @@ int st_field_create(st_ctx *ctx, const st_mesh *mesh, int32_t precision, st_fiel
 
 int st_field_destroy(st_field *f) {
     if (!f) return ST_OK;
+    if (f->pins > 0)
+        return fail(ST_E_STATE, "field is used by %d live graph(s), batch(es) or adjoint(s); destroy them first",
+                    f->pins);
     quiesce_ctx(f->ctx);
     if (f->U) hipFree(f->U);
     if (f->src) hipFree(f->src);
@@ -718,12 +841,14 @@ int st_evolve(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int32_t ns
         ST_TRY(check_tile(ctx, c->prec, p.brick));
         ST_TRY(maybe_sort(c, f, p));
     }
+    StatsBuf sb;
+    ST_TRY(cloud_stats(c, stats_slots(c, p), sb));
     if (c->prec == ST_FP64)
-        ST_TRY(run_passes<double>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, false));
+        ST_TRY(run_passes<double>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, false, sb, true));
     else
-        ST_TRY(run_passes<float>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, false));
+        ST_TRY(run_passes<float>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, false, sb, true));
     if (bbox_out) {
-        ST_HIP(hipMemcpyAsync(bbox_out, c->bbox, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        ST_HIP(hipMemcpyAsync(bbox_out, sb.bbox, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
     }
     return ST_OK;
@@ -738,6 +863,16 @@ int st_cloud_stray_fraction(st_cloud *c, double *fraction) {
         c->last_stray_fraction = c->strays_den > 0 ? (double)*c->strays_host / (double)c->strays_den : 0.0;
     }
     *fraction = c->last_stray_fraction;
+    return ST_OK;
+}
+
+int st_cloud_resort_count(st_cloud *c, int64_t *count) {
+    if (!c || !count) return fail(ST_E_INVALID, "null argument");
+    ST_TRY(check_ctx(c->ctx));
+    unsigned int v = 0;
+    ST_HIP(hipMemcpyAsync(&v, c->resort + 1, sizeof v, hipMemcpyDeviceToHost, c->ctx->stream));
+    ST_HIP(hipStreamSynchronize(c->ctx->stream));
+    *count = (int64_t)v;
     return ST_OK;
 }
 
@@ -759,12 +894,14 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
     if (p.method == ST_METHOD_SORTED && !sort_matches(c, f, p))
         ST_TRY(c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p));
     // every allocation happens before the capture: none is permitted inside it
-    ST_TRY(ensure_part(c, p.method == ST_METHOD_SORTED ? c->nbricks : (int32_t)((c->n + p.block - 1) / p.block)));
+    StatsBuf sb;
+    ST_TRY(cloud_stats(c, stats_slots(c, p), sb));
     ST_HIP(hipStreamSynchronize(ctx->stream));
     st_graph *g = new (std::nothrow) st_graph;
     if (!g) return fail(ST_E_NOMEM, "host allocation failed");
     g->ctx = ctx;
     g->field = f;
+    g->cloud = c;
     g->owner_flush = p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER;
     hipStream_t s = ctx->stream;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -779,8 +916,9 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
         if (e != hipSuccess) r = fail(ST_E_DEVICE, "capture fill: %s", hipGetErrorString(e));
     }
     if (r == ST_OK)
-        r = c->prec == ST_FP64 ? run_passes<double>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, true)
-                               : run_passes<float>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, true);
+        r = c->prec == ST_FP64
+                ? run_passes<double>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, true, sb, true)
+                : run_passes<float>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, true, sb, true);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(s, &graph);
     if (r != ST_OK || e != hipSuccess) {
@@ -796,6 +934,8 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
     }
     g->graph = graph;
     retain_ctx(ctx);
+    pin(c);
+    pin(f);
     *out = g;
     return ST_OK;
 }
@@ -808,6 +948,8 @@ int st_graph_launch(st_graph *g) {
         if (g->field) g->field->stray_dirty = true;
         for (st_field *f : g->more_fields) f->stray_dirty = true;
     }
+    if (g->cloud) ++g->cloud->epoch;
+    for (st_cloud *c : g->more_clouds) ++c->epoch;
     return ST_OK;
 }
 
@@ -816,6 +958,11 @@ int st_graph_destroy(st_graph *g) {
     quiesce_ctx(g->ctx);
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
+    if (g->cloud) unpin(g->cloud);
+    if (g->field) unpin(g->field);
+    for (st_cloud *c : g->more_clouds) unpin(c);
+    for (st_field *f : g->more_fields) unpin(f);
+    if (g->batch) batch_unpin(g->batch);
     st_ctx *ctx = g->ctx;
     delete g;
     release_ctx(ctx);

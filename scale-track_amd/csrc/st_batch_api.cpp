@@ -35,7 +35,16 @@ struct st_batch {
     int grid[2] = {0, 0};            // persistent grid of each group (0: group empty)
     size_t lds = 0;
     double dt_sub = -1.0;            // substep the device table holds
+    int pins = 0;                    // live graphs of this batch
+    bool pinned_members = false;     // the clouds and fields are pinned by this batch
 };
+
+namespace st {
+namespace host {
+void batch_pin(st_batch *b) { ++b->pins; }
+void batch_unpin(st_batch *b) { --b->pins; }
+}  // namespace host
+}  // namespace st
 
 namespace {
 
@@ -175,7 +184,12 @@ int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
 
 int destroy(st_batch *b) {
     if (!b) return ST_OK;
+    if (b->pins > 0) return fail(ST_E_STATE, "batch is used by %d live graph(s); destroy them first", b->pins);
     quiesce_ctx(b->ctx);
+    if (b->pinned_members) {
+        for (st_cloud *c : b->clouds) unpin(c);
+        for (st_field *f : b->fields) unpin(f);
+    }
     if (b->descs) (void)hipFree(b->descs);
     if (b->item0) (void)hipFree(b->item0);
     if (b->targets) (void)hipFree(b->targets);
@@ -283,6 +297,9 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
         destroy(b);
         return r;
     }
+    for (st_cloud *c : b->clouds) pin(c);
+    for (st_field *f : b->fields) pin(f);
+    b->pinned_members = true;
     *out = b;
     return ST_OK;
 }
@@ -316,6 +333,7 @@ int st_batch_graph_capture(st_batch *b, double dt_fluid, int32_t nsub, int32_t z
     if (!g) return fail(ST_E_NOMEM, "host allocation failed");
     g->ctx = b->ctx;
     g->more_fields = b->fields;
+    g->more_clouds = b->clouds;
     g->owner_flush = b->p.flush == ST_FLUSH_OWNER;
     hipStream_t s = b->ctx->stream;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -350,6 +368,10 @@ int st_batch_graph_capture(st_batch *b, double dt_fluid, int32_t nsub, int32_t z
     }
     g->graph = graph;
     retain_ctx(b->ctx);
+    g->batch = b;
+    batch_pin(b);
+    for (st_cloud *c : b->clouds) pin(c);
+    for (st_field *f : b->fields) pin(f);
     *out = g;
     return ST_OK;
 }

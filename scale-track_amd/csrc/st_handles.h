@@ -46,6 +46,7 @@ struct st_ctx {
     int64_t tcount = 0;                  // timed calls so far (slot = index % kTimerRing)
     size_t max_lds = 65536;      // LDS bytes one workgroup may use on this device
     size_t lds_per_cu = 163840;  // LDS bytes of one CU (all resident workgroups together)
+    int cus = 256;               // compute units of the device
     // Clouds, fields, graphs, batches and adjoints hold a reference: st_ctx_destroy only
     // marks the context closed while any of them is alive; the last release frees it.
     int refs = 0;
@@ -89,6 +90,16 @@ struct st_cloud {
     bool strays_pending = false;
     int64_t strays_den = 0;  // parcels x passes behind strays_host
     double last_stray_fraction = -1.0;
+    // device: [0] gate of the re-sort queued behind the last step (set by the statistics
+    // reduction), [1] re-sorts the device decided on so far
+    unsigned int *resort = nullptr;
+    // Graphs, batches and adjoints holding raw device pointers of this cloud: while any
+    // is alive the cloud refuses to be destroyed, re-sorted with another brick geometry,
+    // or to grow the buffers they captured.
+    int pins = 0;
+    // Bumped whenever the device order or state of the parcels may have changed (upload,
+    // sort, step, queued re-sort): an adjoint checks it between its forward and backward.
+    int64_t epoch = 0;
 };
 
 struct st_field {
@@ -103,12 +114,18 @@ struct st_field {
     double *stray = nullptr;
     bool U_set = false;
     bool stray_dirty = false;
+    int pins = 0;  // live graphs, batches and adjoints using this field
 };
+
+struct st_batch;
 
 struct st_graph {
     st_ctx *ctx = nullptr;
     st_field *field = nullptr;
+    st_cloud *cloud = nullptr;
     std::vector<st_field *> more_fields;  // batch graphs: every field the graph deposits into
+    std::vector<st_cloud *> more_clouds;  // batch graphs: every cloud the graph steps
+    st_batch *batch = nullptr;            // batch graphs: the batch whose device tables they use
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool owner_flush = false;
@@ -130,12 +147,35 @@ int validate_params(const st_params &p);
 bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p);
 int sort_any(st_cloud *c, st_field *f, const st_params &p);
 int prepare(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int nsub, const st_params *pin, st_params &p);
+// Where a step's statistics go: per-workgroup slots (cap of them), the reduced bounding
+// box and stray count, and the gate of the device-side re-sort decision.
+struct StatsBuf {
+    double *part = nullptr;
+    int32_t cap = 0;
+    double *bbox = nullptr;
+    unsigned int *strays = nullptr;
+    unsigned int *gate = nullptr;
+};
+// The cloud's own statistics buffers (slots grown to nslots on demand).
+int cloud_stats(st_cloud *c, int32_t nslots, StatsBuf &sb);
+// Statistics slots one step of the cloud uses with these params.
+int32_t stats_slots(const st_cloud *c, const st_params &p);
 // Queues nsub substeps of length dt_sub (exact in T) on the context stream, split in
 // passes of p.substeps_per_pass.  With ck (sorted method, FP64) the first pass also
 // writes the state each parcel starts with into ck (6 arrays of c->n, device order).
+// With device_resort (sorted method) a re-sort of the cloud is queued behind the step,
+// gated on the share of parcels found outside their brick exceeding p.resort_fraction.
 template <class T>
 int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p, bool capturing,
-               T *ck = nullptr);
+               const StatsBuf &sb, bool device_resort, T *ck = nullptr);
+// Pins and unpins a batch (graphs of it hold its device tables; st_batch_api.cpp).
+void batch_pin(st_batch *b);
+void batch_unpin(st_batch *b);
+// Pins and unpins a cloud / field (see st_cloud::pins).
+void pin(st_cloud *c);
+void unpin(st_cloud *c);
+void pin(st_field *f);
+void unpin(st_field *f);
 // The substep length the reference uses: Time stores dt as `scalar` (:272-275) and the
 // kernel divides by nSteps (:1261).
 inline double substep_dt(int prec, double dt_fluid, int nsub) {

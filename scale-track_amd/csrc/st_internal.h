@@ -48,12 +48,17 @@ struct Soa {
 // number of parcels found outside their brick.
 constexpr int kStatSlot = 8;  // doubles per slot
 
-// Where one cloud's per-slot statistics are reduced to.
+// Where one cloud's per-slot statistics are reduced to.  With `resort` set, the
+// reduction also writes resort[0] = 1 when more than `threshold` parcels were outside
+// their brick (else 0): the gate of the re-sort queued behind the pass, so the
+// decision is taken on the device without a host round trip.
 struct StatsTarget {
     const double *part;  // nslots * kStatSlot
     int32_t nslots;
     double *bbox;        // 6: min xyz, max xyz
     unsigned int *strays;
+    unsigned int *resort;  // or nullptr
+    uint32_t threshold;
 };
 
 template <class T>
@@ -103,6 +108,12 @@ template <class T>
 hipError_t launch_pass_sorted(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
 template <class T>
 hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
+// npass passes of a.nsteps substeps each in one persistent launch of `grid` workgroups
+// (each owning the same bricks in every pass); a.part is filled in the last pass.
+template <class T>
+hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, int periodic, int block, int grid, hipStream_t s);
+template <class T>
+int pass_multi_blocks_per_cu(int periodic, int rho, int block, size_t lds);
 // Persistent pass over clouds [0, ncloud) of descs: work items item0[0] .. item0[ncloud]-1
 // (counter must hold item0[0] at launch); rho: every one of these clouds carries a
 // per-parcel density array (else none does).
@@ -114,26 +125,34 @@ template <class T>
 int batch_max_blocks_per_cu(int periodic, int rho, int block, size_t lds);
 hipError_t launch_reduce_stats(const StatsTarget &t, hipStream_t s);
 hipError_t launch_reduce_stats_many(const StatsTarget *targets, int32_t n, hipStream_t s);
+// Every kernel of the sort takes a gate: nullptr runs it; otherwise it does nothing
+// unless gate[0] != 0 (the device-side re-sort decision of StatsTarget::resort).
 template <class T>
 hipError_t launch_brick_keys(const Soa<T> &s, uint32_t np, Geo g, const int32_t nb[3],
                              const int32_t bd[3], int periodic, uint32_t *keys,
-                             uint32_t *counts, unsigned int *bad, hipStream_t st);
+                             uint32_t *counts, unsigned int *bad, const unsigned int *gate, hipStream_t st);
 // Counting sort by brick in per-workgroup chunks (nbins * 4 B of LDS per workgroup):
 // keys, per-chunk histograms (nrows x nbins in hist), brick starts and the scatter.
 template <class T>
 hipError_t launch_sort_chunked(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out, uint32_t np,
                                Geo g, const int32_t nb[3], const int32_t bd[3], int periodic, int32_t nrows,
                                uint32_t *keys, uint32_t *hist, uint32_t *counts, int32_t *start,
-                               uint32_t *cursor, unsigned int *bad, hipStream_t s);
+                               uint32_t *cursor, unsigned int *bad, const unsigned int *gate, hipStream_t s);
 hipError_t launch_scan(const uint32_t *counts, int32_t nbins, int32_t *start, uint32_t *cursor,
-                       hipStream_t s);
+                       const unsigned int *gate, hipStream_t s);
 template <class T>
 hipError_t launch_scatter(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out,
-                          const uint32_t *keys, uint32_t *cursor, uint32_t np, hipStream_t s);
+                          const uint32_t *keys, uint32_t *cursor, uint32_t np, const unsigned int *gate,
+                          hipStream_t s);
+// Copies the sorted set back over the cloud's own arrays (so device pointers never
+// change), gated like the sort; a gated copy that runs adds one to *count.
+template <class T>
+hipError_t launch_copy_back(const Soa<T> &from, const int32_t *pid_from, Soa<T> to, int32_t *pid_to, uint32_t np,
+                            const unsigned int *gate, unsigned int *count, hipStream_t s);
 template <class T>
 hipError_t launch_unpermute(const Soa<T> &in, const int32_t *pid, Soa<T> out, uint32_t np,
                             hipStream_t s);
-hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s, const unsigned int *gate = nullptr);
 hipError_t launch_fill_f64(double *p, double value, size_t n, hipStream_t s);
 hipError_t launch_iota(int32_t *p, uint32_t n, hipStream_t s);
 hipError_t launch_merge(double *src, double *stray, size_t n, hipStream_t s);

@@ -588,6 +588,60 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeo
     process_brick<T, PERIODIC, RHO, CKPT>(a, blockIdx.x, a.nsteps, lds, a.part);
 }
 
+// Several passes of one sorted cloud in one persistent launch.  Workgroup w owns bricks
+// w, w + gridDim.x, ... in every pass and runs its bricks pass by pass, so a brick's
+// pass k + 1 follows its pass k in the same workgroup: the parcels of a brick, and its
+// own cells of `src` (owner flush), are read and written by that workgroup alone, with
+// its barriers ordering them, and everything shared (halo and stray sums) goes to
+// `stray` with atomics.  Pass-major order keeps each pass a full sweep over the
+// parcel arrays in HBM.  `part` (statistics) is filled in the last pass only.
+template <class T, bool PERIODIC, bool RHO>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4)))
+k_pass_multi(PassArgs<T> a, int32_t npass) {
+    extern __shared__ double lds[];
+    for (int32_t k = 0; k < npass; ++k) {
+        double *part = k == npass - 1 ? a.part : nullptr;
+        for (int32_t brick = blockIdx.x; brick < a.nbricks; brick += gridDim.x) {
+            process_brick<T, PERIODIC, RHO>(a, brick, a.nsteps, lds, part);
+            __syncthreads();  // the flush's tile reads are done before the next staging
+        }
+    }
+}
+
+template <class T>
+hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, int periodic, int block, int grid, hipStream_t s) {
+    const size_t lds = (size_t)a.tile_cells * 3 * (sizeof(double) + sizeof(T));
+    const bool rho = a.s.rho != nullptr;
+    if (periodic && rho)
+        k_pass_multi<T, true, true><<<grid, block, lds, s>>>(a, npass);
+    else if (periodic)
+        k_pass_multi<T, true, false><<<grid, block, lds, s>>>(a, npass);
+    else if (rho)
+        k_pass_multi<T, false, true><<<grid, block, lds, s>>>(a, npass);
+    else
+        k_pass_multi<T, false, false><<<grid, block, lds, s>>>(a, npass);
+    return hipGetLastError();
+}
+template hipError_t launch_pass_multi<float>(const PassArgs<float> &, int32_t, int, int, int, hipStream_t);
+template hipError_t launch_pass_multi<double>(const PassArgs<double> &, int32_t, int, int, int, hipStream_t);
+
+template <class T>
+int pass_multi_blocks_per_cu(int periodic, int rho, int block, size_t lds) {
+    int n = 0;
+    hipError_t e;
+    if (periodic && rho)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_multi<T, true, true>, block, lds);
+    else if (periodic)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_multi<T, true, false>, block, lds);
+    else if (rho)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_multi<T, false, true>, block, lds);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_multi<T, false, false>, block, lds);
+    return e == hipSuccess ? n : 0;
+}
+template int pass_multi_blocks_per_cu<float>(int, int, int, size_t);
+template int pass_multi_blocks_per_cu<double>(int, int, int, size_t);
+
 // Many clouds in one persistent launch: each workgroup takes (cloud, brick) work items
 // from a global counter until none are left.  item0[c] is the first item of cloud c
 // (clouds ordered densest first, so the longest items are handed out first); every
@@ -750,7 +804,9 @@ __device__ __forceinline__ void reduce_stats(const StatsTarget &t) {
             v[6] += red[6][w];
         }
         for (int c = 0; c < 6; ++c) t.bbox[c] = v[c];
-        t.strays[0] = (unsigned int)v[6];
+        const unsigned int strays = (unsigned int)v[6];
+        t.strays[0] = strays;
+        if (t.resort) t.resort[0] = strays > t.threshold ? 1u : 0u;
     }
 }
 
@@ -768,7 +824,12 @@ hipError_t launch_reduce_stats_many(const StatsTarget *targets, int32_t n, hipSt
     return hipGetLastError();
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+// A gated kernel does nothing unless its gate is set (the gate is uniform: every
+// thread returns or none does).
+__device__ __forceinline__ bool gated_off(const unsigned int *gate) { return gate && gate[0] == 0u; }
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n, const unsigned int *gate) {
+    if (gated_off(gate)) return;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
@@ -797,9 +858,9 @@ static unsigned grid_for(size_t n, unsigned block) {
     return (unsigned)g;
 }
 
-hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s, const unsigned int *gate) {
     if (!n) return hipSuccess;
-    k_fill_u32<<<grid_for(n, 256), 256, 0, s>>>(p, v, n);
+    k_fill_u32<<<grid_for(n, 256), 256, 0, s>>>(p, v, n, gate);
     return hipGetLastError();
 }
 hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s) {
@@ -853,7 +914,8 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t *ctr, uint32_t key, bool
 template <class T, bool PERIODIC>
 __global__ void k_brick_keys(Soa<T> s, uint32_t np, Geo g, int32_t nb0, int32_t nb1, int32_t bd0,
                              int32_t bd1, int32_t bd2, uint32_t *keys, uint32_t *counts,
-                             unsigned int *bad) {
+                             unsigned int *bad, const unsigned int *gate) {
+    if (gated_off(gate)) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = p < np;
     uint32_t key = 0;
@@ -872,29 +934,30 @@ __global__ void k_brick_keys(Soa<T> s, uint32_t np, Geo g, int32_t nb0, int32_t 
 template <class T>
 hipError_t launch_brick_keys(const Soa<T> &s, uint32_t np, Geo g, const int32_t nb[3],
                              const int32_t bd[3], int periodic, uint32_t *keys, uint32_t *counts,
-                             unsigned int *bad, hipStream_t st) {
+                             unsigned int *bad, const unsigned int *gate, hipStream_t st) {
     if (!np) return hipSuccess;
     const uint32_t grid = (np + 255) / 256;
     if (periodic)
         k_brick_keys<T, true><<<grid, 256, 0, st>>>(s, np, g, nb[0], nb[1], bd[0], bd[1], bd[2], keys,
-                                                     counts, bad);
+                                                     counts, bad, gate);
     else
         k_brick_keys<T, false><<<grid, 256, 0, st>>>(s, np, g, nb[0], nb[1], bd[0], bd[1], bd[2], keys,
-                                                      counts, bad);
+                                                      counts, bad, gate);
     return hipGetLastError();
 }
 template hipError_t launch_brick_keys<float>(const Soa<float> &, uint32_t, Geo, const int32_t *,
                                              const int32_t *, int, uint32_t *, uint32_t *,
-                                             unsigned int *, hipStream_t);
+                                             unsigned int *, const unsigned int *, hipStream_t);
 template hipError_t launch_brick_keys<double>(const Soa<double> &, uint32_t, Geo, const int32_t *,
                                               const int32_t *, int, uint32_t *, uint32_t *,
-                                              unsigned int *, hipStream_t);
+                                              unsigned int *, const unsigned int *, hipStream_t);
 
 // Exclusive scan of the bin counts in one workgroup of 1024 threads: each thread
 // sums a contiguous segment, the 1024 partial sums are scanned in LDS, then each
 // thread writes its segment.  start has nbins + 1 entries; cursor = start[0..nbins).
 __global__ void __launch_bounds__(1024) k_scan(const uint32_t *counts, int32_t nbins, int32_t *start,
-                                               uint32_t *cursor) {
+                                               uint32_t *cursor, const unsigned int *gate) {
+    if (gated_off(gate)) return;
     __shared__ uint32_t part[1024];
     const int32_t t = threadIdx.x;
     const int32_t seg = (nbins + 1023) / 1024;
@@ -919,8 +982,8 @@ __global__ void __launch_bounds__(1024) k_scan(const uint32_t *counts, int32_t n
 }
 
 hipError_t launch_scan(const uint32_t *counts, int32_t nbins, int32_t *start, uint32_t *cursor,
-                       hipStream_t s) {
-    k_scan<<<1, 1024, 0, s>>>(counts, nbins, start, cursor);
+                       const unsigned int *gate, hipStream_t s) {
+    k_scan<<<1, 1024, 0, s>>>(counts, nbins, start, cursor, gate);
     return hipGetLastError();
 }
 
@@ -936,7 +999,9 @@ template <class T, bool PERIODIC>
 __global__ void __launch_bounds__(kSortBlock) k_sort_count(Soa<T> s, uint32_t np, uint32_t chunk, Geo g,
                                                            int32_t nb0, int32_t nb1, int32_t bd0, int32_t bd1,
                                                            int32_t bd2, int32_t nbins, uint32_t *keys,
-                                                           uint32_t *hist, unsigned int *bad) {
+                                                           uint32_t *hist, unsigned int *bad,
+                                                           const unsigned int *gate) {
+    if (gated_off(gate)) return;
     extern __shared__ uint32_t h[];
     for (int32_t b = threadIdx.x; b < nbins; b += kSortBlock) h[b] = 0u;
     __syncthreads();
@@ -960,7 +1025,9 @@ __global__ void __launch_bounds__(kSortBlock) k_sort_count(Soa<T> s, uint32_t np
 
 // Per brick: the exclusive prefix of its counts over the workgroups (in place) and the
 // total (into counts, for k_scan).
-__global__ void k_col_prefix(uint32_t *__restrict__ hist, int32_t nbins, int32_t nrows, uint32_t *counts) {
+__global__ void k_col_prefix(uint32_t *__restrict__ hist, int32_t nbins, int32_t nrows, uint32_t *counts,
+                             const unsigned int *gate) {
+    if (gated_off(gate)) return;
     const int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbins) return;
     constexpr int kBatch = 16;  // loads in flight per thread
@@ -982,7 +1049,9 @@ template <class T>
 __global__ void __launch_bounds__(kSortBlock) k_sort_scatter(Soa<T> in, const int32_t *pid_in, Soa<T> out,
                                                              int32_t *pid_out, const uint32_t *keys,
                                                              const uint32_t *hist, const int32_t *start,
-                                                             int32_t nbins, uint32_t np, uint32_t chunk) {
+                                                             int32_t nbins, uint32_t np, uint32_t chunk,
+                                                             const unsigned int *gate) {
+    if (gated_off(gate)) return;
     extern __shared__ uint32_t cur[];
     const uint32_t *row = hist + (size_t)blockIdx.x * nbins;
     for (int32_t b = threadIdx.x; b < nbins; b += kSortBlock) cur[b] = (uint32_t)start[b] + row[b];
@@ -1033,35 +1102,36 @@ template <class T>
 hipError_t launch_sort_chunked(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out, uint32_t np,
                                Geo g, const int32_t nb[3], const int32_t bd[3], int periodic, int32_t nrows,
                                uint32_t *keys, uint32_t *hist, uint32_t *counts, int32_t *start,
-                               uint32_t *cursor, unsigned int *bad, hipStream_t s) {
+                               uint32_t *cursor, unsigned int *bad, const unsigned int *gate, hipStream_t s) {
     const int32_t nbins = nb[0] * nb[1] * nb[2];
     const uint32_t chunk = (np + (uint32_t)nrows - 1) / (uint32_t)nrows;
     const size_t lds = (size_t)nbins * 4;
     if (periodic)
         k_sort_count<T, true><<<nrows, kSortBlock, lds, s>>>(in, np, chunk, g, nb[0], nb[1], bd[0], bd[1], bd[2],
-                                                             nbins, keys, hist, bad);
+                                                             nbins, keys, hist, bad, gate);
     else
         k_sort_count<T, false><<<nrows, kSortBlock, lds, s>>>(in, np, chunk, g, nb[0], nb[1], bd[0], bd[1], bd[2],
-                                                              nbins, keys, hist, bad);
-    k_col_prefix<<<grid_for((size_t)nbins, 256), 256, 0, s>>>(hist, nbins, nrows, counts);
-    k_scan<<<1, 1024, 0, s>>>(counts, nbins, start, cursor);
+                                                              nbins, keys, hist, bad, gate);
+    k_col_prefix<<<grid_for((size_t)nbins, 256), 256, 0, s>>>(hist, nbins, nrows, counts, gate);
+    k_scan<<<1, 1024, 0, s>>>(counts, nbins, start, cursor, gate);
     if (np) k_sort_scatter<T><<<nrows, kSortBlock, lds, s>>>(in, pid_in, out, pid_out, keys, hist, start, nbins, np,
-                                                             chunk);
+                                                             chunk, gate);
     return hipGetLastError();
 }
 template hipError_t launch_sort_chunked<float>(const Soa<float> &, const int32_t *, Soa<float>, int32_t *, uint32_t,
                                                Geo, const int32_t[3], const int32_t[3], int, int32_t, uint32_t *,
                                                uint32_t *, uint32_t *, int32_t *, uint32_t *, unsigned int *,
-                                               hipStream_t);
+                                               const unsigned int *, hipStream_t);
 template hipError_t launch_sort_chunked<double>(const Soa<double> &, const int32_t *, Soa<double>, int32_t *,
                                                 uint32_t, Geo, const int32_t[3], const int32_t[3], int, int32_t,
                                                 uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *,
-                                                unsigned int *, hipStream_t);
+                                                unsigned int *, const unsigned int *, hipStream_t);
 
 
 template <class T>
 __global__ void k_scatter(Soa<T> in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out,
-                          const uint32_t *keys, uint32_t *cursor, uint32_t np) {
+                          const uint32_t *keys, uint32_t *cursor, uint32_t np, const unsigned int *gate) {
+    if (gated_off(gate)) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = p < np;
     const uint32_t key = active ? keys[p] : 0u;
@@ -1080,16 +1150,74 @@ __global__ void k_scatter(Soa<T> in, const int32_t *pid_in, Soa<T> out, int32_t 
 
 template <class T>
 hipError_t launch_scatter(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out,
-                          const uint32_t *keys, uint32_t *cursor, uint32_t np, hipStream_t s) {
+                          const uint32_t *keys, uint32_t *cursor, uint32_t np, const unsigned int *gate,
+                          hipStream_t s) {
     if (!np) return hipSuccess;
-    k_scatter<T><<<(np + 255) / 256, 256, 0, s>>>(in, pid_in, out, pid_out, keys, cursor, np);
+    k_scatter<T><<<(np + 255) / 256, 256, 0, s>>>(in, pid_in, out, pid_out, keys, cursor, np, gate);
     return hipGetLastError();
 }
 template hipError_t launch_scatter<float>(const Soa<float> &, const int32_t *, Soa<float>, int32_t *,
-                                          const uint32_t *, uint32_t *, uint32_t, hipStream_t);
+                                          const uint32_t *, uint32_t *, uint32_t, const unsigned int *,
+                                          hipStream_t);
 template hipError_t launch_scatter<double>(const Soa<double> &, const int32_t *, Soa<double>,
                                            int32_t *, const uint32_t *, uint32_t *, uint32_t,
-                                           hipStream_t);
+                                           const unsigned int *, hipStream_t);
+
+// The sorted set copied back over the cloud's arrays, four elements per thread per
+// round with every load issued before the first store.
+template <class T>
+__global__ void __launch_bounds__(256) k_copy_back(Soa<T> from, const int32_t *pid_from, Soa<T> to, int32_t *pid_to,
+                                                   uint32_t np, const unsigned int *gate, unsigned int *count) {
+    if (gated_off(gate)) return;
+    if (gate && count && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(count, 1u);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const bool rho = from.rho != nullptr;
+    constexpr int kU = 4;
+    for (uint32_t p0 = blockIdx.x * blockDim.x + threadIdx.x; p0 < np; p0 += kU * stride) {
+        T d[kU], x[kU], y[kU], z[kU], u[kU], v[kU], w[kU], r[kU];
+        int32_t id[kU];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const uint32_t p = min(p0 + k * stride, np - 1);
+            d[k] = from.d[p];
+            x[k] = from.x[p];
+            y[k] = from.y[p];
+            z[k] = from.z[p];
+            u[k] = from.u[p];
+            v[k] = from.v[p];
+            w[k] = from.w[p];
+            r[k] = rho ? from.rho[p] : (T)0;
+            id[k] = pid_from[p];
+        }
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const uint32_t p = p0 + k * stride;
+            if (p >= np) continue;
+            to.d[p] = d[k];
+            to.x[p] = x[k];
+            to.y[p] = y[k];
+            to.z[p] = z[k];
+            to.u[p] = u[k];
+            to.v[p] = v[k];
+            to.w[p] = w[k];
+            if (rho) ((T *)to.rho)[p] = r[k];
+            pid_to[p] = id[k];
+        }
+    }
+}
+
+template <class T>
+hipError_t launch_copy_back(const Soa<T> &from, const int32_t *pid_from, Soa<T> to, int32_t *pid_to, uint32_t np,
+                            const unsigned int *gate, unsigned int *count, hipStream_t s) {
+    if (!np) return hipSuccess;
+    const unsigned grid = grid_for(((size_t)np + 3) / 4, 256);
+    k_copy_back<T><<<grid, 256, 0, s>>>(from, pid_from, to, pid_to, np, gate, count);
+    return hipGetLastError();
+}
+template hipError_t launch_copy_back<float>(const Soa<float> &, const int32_t *, Soa<float>, int32_t *, uint32_t,
+                                            const unsigned int *, unsigned int *, hipStream_t);
+template hipError_t launch_copy_back<double>(const Soa<double> &, const int32_t *, Soa<double>, int32_t *,
+                                             uint32_t, const unsigned int *, unsigned int *, hipStream_t);
 
 template <class T>
 __global__ void k_unpermute(Soa<T> in, const int32_t *pid, Soa<T> out, uint32_t np) {

@@ -50,6 +50,7 @@ class st_params(C.Structure):
         ("brick", C.c_int32 * 3),
         ("block", C.c_int32),
         ("resort_fraction", C.c_double),
+        ("passes_per_launch", C.c_int32),
     ]
 
 
@@ -98,6 +99,7 @@ SIGNATURES = {
                             C.POINTER(C.c_double)]),
     "st_cloud_stray_fraction": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "st_cloud_sort": (C.c_int, [_vp, _vp, _vp, C.POINTER(st_params)]),
+    "st_cloud_resort_count": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "st_graph_capture": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int32, C.POINTER(st_params),
                                    C.c_int32, _pp]),
     "st_graph_launch": (C.c_int, [_vp]),

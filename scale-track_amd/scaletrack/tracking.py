@@ -168,7 +168,7 @@ class TwoWayEulerian:
 
     def close(self):
         if self.handle:
-            lib.st_field_destroy(self.handle)
+            check(lib.st_field_destroy(self.handle))
             self.handle = None
 
     def __del__(self):
@@ -225,9 +225,15 @@ class Chunk:
         check(lib.st_cloud_stray_fraction(self.handle, C.byref(f)))
         return f.value
 
+    def resort_count(self):
+        """Re-sorts the device decided on so far (st_cloud_resort_count)."""
+        n = C.c_int64()
+        check(lib.st_cloud_resort_count(self.handle, C.byref(n)))
+        return n.value
+
     def close(self):
         if self.handle:
-            lib.st_cloud_destroy(self.handle)
+            check(lib.st_cloud_destroy(self.handle))
             self.handle = None
 
     def __del__(self):
@@ -278,6 +284,7 @@ class Graph:
                                    int(nsub), C.byref(p) if p is not None else None,
                                    int(bool(zero_source)), C.byref(h)))
         self.handle = h
+        self._keep = (chunk, eulerian)  # the library refuses to free them while the graph lives
 
     def launch(self):
         check(lib.st_graph_launch(self.handle))
@@ -334,11 +341,12 @@ class Batch:
         check(lib.st_batch_graph_capture(self.handle, float(dt_fluid), int(nsub), int(bool(zero_source)),
                                          C.byref(h)))
         g.handle = h
+        g._keep = (self,)
         return g
 
     def close(self):
         if self.handle:
-            lib.st_batch_destroy(self.handle)
+            check(lib.st_batch_destroy(self.handle))
             self.handle = None
 
     def __del__(self):

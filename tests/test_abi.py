@@ -31,11 +31,12 @@ def test_header_symbols_exported():
 
 
 def test_version_and_defaults():
-    assert _lib.lib.st_version() == 1
+    assert _lib.lib.st_version() == 2
     p = _lib.default_params()
     assert (p.boundary, p.method, p.flush, p.substeps_per_pass) == (0, 1, 0, 0)
     assert p.cd == 1.0 and list(p.brick) == [32, 8, 2] and p.block == 512
     assert 0 < p.resort_fraction < 1
+    assert p.passes_per_launch == 1
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -45,8 +46,9 @@ def test_struct_layouts_match_header(tmp_path):
         '#include <stdio.h>\n#include <stddef.h>\n#include "scaletrack.h"\n'
         "int main(void) {\n"
         '  printf("%zu %zu %zu\\n", sizeof(st_mesh), offsetof(st_mesh, origin), offsetof(st_mesh, ending));\n'
-        '  printf("%zu %zu %zu %zu %zu\\n", sizeof(st_params), offsetof(st_params, cd),\n'
-        "         offsetof(st_params, brick), offsetof(st_params, block), offsetof(st_params, resort_fraction));\n"
+        '  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(st_params), offsetof(st_params, cd),\n'
+        "         offsetof(st_params, brick), offsetof(st_params, block), offsetof(st_params, resort_fraction),\n"
+        "         offsetof(st_params, passes_per_launch));\n"
         "  return 0;\n}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -54,7 +56,7 @@ def test_struct_layouts_match_header(tmp_path):
     M, P = _lib.st_mesh, _lib.st_params
     assert [int(v) for v in l1.split()] == [C.sizeof(M), M.origin.offset, M.ending.offset]
     assert [int(v) for v in l2.split()] == [C.sizeof(P), P.cd.offset, P.brick.offset, P.block.offset,
-                                            P.resort_fraction.offset]
+                                            P.resort_fraction.offset, P.passes_per_launch.offset]
 
 
 def test_null_arguments_are_rejected():
