@@ -146,6 +146,19 @@ int st_field_download_source(st_field *field, void *out, int32_t elem_bytes);
 int st_evolve(st_ctx *ctx, st_cloud *cloud, st_field *field, double dt_fluid, int32_t nsub,
               const st_params *params, double bbox_out[6]);
 
+/* No counterpart in the reference.  st_evolve for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same step, with its statistics slots, bounding box, stray count and
+ * re-sort gate in one fixed library-owned device array of 4 MiB + 256 B per GPU instead
+ * of buffers grown on demand.  Limits: at most 65536 statistics slots (one per brick
+ * with the sorted method, one per workgroup with the plain one; ST_E_RANGE above), and
+ * with the sorted method the cloud must already be sorted for this mesh and brick
+ * (st_cloud_sort or an earlier st_evolve; ST_E_STATE otherwise).  Every call on one GPU
+ * shares that array and it keeps the last call's contents: calls that overlap on one
+ * GPU (two contexts or threads) race. */
+int st_evolve_noalloc(st_ctx *ctx, st_cloud *cloud, st_field *field, double dt_fluid, int32_t nsub,
+                      const st_params *params, double bbox_out[6]);
+
 /* Share of parcels that were outside their sorted brick during the last sorted pass
  * (synchronises); -1 if unknown. */
 int st_cloud_stray_fraction(st_cloud *cloud, double *fraction);

@@ -549,6 +549,9 @@ int st_ctx_create(int32_t device, st_ctx **out) {
         lds > 0)
         c->cus = lds;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    // st_evolve_noalloc's scratch address on this device (a failed lookup only disables
+    // that entry point, which then says so)
+    if (e == hipSuccess) (void)noalloc_setup(device);
     for (int k = 0; k < kTimerRing && e == hipSuccess; ++k) {
         e = hipEventCreate(&c->tev[k][0]);
         if (e == hipSuccess) e = hipEventCreate(&c->tev[k][1]);

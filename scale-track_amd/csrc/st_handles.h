@@ -168,6 +168,9 @@ int32_t stats_slots(const st_cloud *c, const st_params &p);
 template <class T>
 int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p, bool capturing,
                const StatsBuf &sb, bool device_resort, T *ck = nullptr);
+// Looks up the device's copy of the no-allocation entry point's statistics scratch
+// (st_noalloc.hip); called by st_ctx_create with the device current.
+int noalloc_setup(int device);
 // Pins and unpins a batch (graphs of it hold its device tables; st_batch_api.cpp).
 void batch_pin(st_batch *b);
 void batch_unpin(st_batch *b);

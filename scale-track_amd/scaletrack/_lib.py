@@ -97,6 +97,8 @@ SIGNATURES = {
     "st_field_download_source": (C.c_int, [_vp, _vp, C.c_int32]),
     "st_evolve": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int32, C.POINTER(st_params),
                             C.POINTER(C.c_double)]),
+    "st_evolve_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int32, C.POINTER(st_params),
+                                    C.POINTER(C.c_double)]),
     "st_cloud_stray_fraction": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "st_cloud_sort": (C.c_int, [_vp, _vp, _vp, C.POINTER(st_params)]),
     "st_cloud_resort_count": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
