@@ -50,8 +50,15 @@ typedef enum st_status {
  * (x - origin) * rDelta and the momentum source in double. */
 typedef enum st_precision { ST_FP64 = 0, ST_FP32 = 1 } st_precision;
 
-/* ST_WALL: the reference's specular bounce (:1306-1335).  ST_PERIODIC: extension. */
-typedef enum st_boundary { ST_WALL = 0, ST_PERIODIC = 1 } st_boundary;
+/* ST_WALL: the reference's specular bounce (:1306-1335) on every face.  ST_PERIODIC:
+ * extension, every axis periodic.  ST_CHANNEL: extension, walls on y and z while the x
+ * planes at origin and ending are open -- a substep that ends outside them (tested
+ * every substep, before the bounce test) ends the parcel: its residence deposit
+ * m (s0 - v') goes into the cell it was in, its state is stored as advanced, and its
+ * diameter is stored negated, which marks it removed: later steps leave it untouched,
+ * downloads return it with d < 0, st_cloud_compact drops it.  The reference has no
+ * open boundary; its chunks keep every parcel (asyncParallelTracking.jl:329-364). */
+typedef enum st_boundary { ST_WALL = 0, ST_PERIODIC = 1, ST_CHANNEL = 2 } st_boundary;
 
 /* ST_METHOD_PLAIN: reference-shaped -- one thread per parcel in the caller's order,
  * cell velocity gathered from HBM, three FP64 atomics per deposit.
@@ -123,6 +130,20 @@ int st_cloud_upload(st_cloud *cloud, const void *d, const void *x, const void *y
 int st_cloud_download(st_cloud *cloud, void *d, void *x, void *y, void *z, void *u,
                       void *v, void *w);
 int st_cloud_count(const st_cloud *cloud, int64_t *n);
+/* Extension (injection and removal; the reference's chunks keep a fixed N,
+ * asyncParallelTracking.jl:329-364).  st_cloud_reserve grows the device arrays to hold
+ * `capacity` parcels.  st_cloud_inject appends k parcels (host SoA in the cloud's
+ * precision; rho as in st_cloud_upload) after the current ones: they take the next
+ * upload-order indices, so st_cloud_count grows by k and downloads return them last,
+ * and the next sorted step sorts the cloud again.  st_cloud_compact drops the parcels
+ * an open boundary removed (ST_CHANNEL; d < 0), keeping the others in their upload
+ * order with their indices renumbered from 0, and returns their number (may be NULL).
+ * All three synchronise, and refuse (ST_E_STATE) while a graph, batch or adjoint holds
+ * the cloud; the first sorted step after them sorts the cloud. */
+int st_cloud_reserve(st_cloud *cloud, int64_t capacity);
+int st_cloud_inject(st_cloud *cloud, int64_t k, const void *d, const void *x, const void *y,
+                    const void *z, const void *u, const void *v, const void *w, const void *rho);
+int st_cloud_compact(st_cloud *cloud, int64_t *n_live);
 
 /* ---- Eulerian fields: the reference's TwoWayEulerian (:207-220) ------------- */
 int st_field_create(st_ctx *ctx, const st_mesh *mesh, int32_t precision, st_field **out);

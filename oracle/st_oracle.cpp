@@ -145,9 +145,10 @@ struct Consts {
 template <class S>
 using DW = typename Wide<S>::type;
 
-/* The step of one parcel for nsteps substeps: asyncParallelTracking.jl:1271-1365. */
+/* The step of one parcel for nsteps substeps: asyncParallelTracking.jl:1271-1365.
+ * Returns true when the parcel left through an open x plane (channel extension). */
 template <class S>
-void evolve_particle(const st_oracle_args &a, const MeshD &m, const Consts<S> &k, S dia, S rho,
+bool evolve_particle(const st_oracle_args &a, const MeshD &m, const Consts<S> &k, S dia, S rho,
                      DW<S> rdr, Vec3<S> &pos, Vec3<S> &vel, const typename Store<S>::type *Uc,
                      int nsteps, DW<S> *src, int64_t *ev) {
     using D = DW<S>;
@@ -174,6 +175,7 @@ void evolve_particle(const st_oracle_args &a, const MeshD &m, const Consts<S> &k
     }
     Vec3<S> uc{{S(Uc[3 * posI]), S(Uc[3 * posI + 1]), S(Uc[3 * posI + 2])}};
     D s0[3] = {widen(vel.c[0]), widen(vel.c[1]), widen(vel.c[2])};
+    bool exited = false;
 
     for (int t = 1; t <= nsteps; ++t) {
         /* :1294-1296 implicit Euler with Stokes drag */
@@ -203,6 +205,16 @@ void evolve_particle(const st_oracle_args &a, const MeshD &m, const Consts<S> &k
         } else if (!periodic) {
             /* :1303 */
             Loc l = locate_wall(m, val(posNew.c[0]), val(posNew.c[1]), val(posNew.c[2]));
+            if (a.boundary == ST_O_CHANNEL && (l.ijk[0] < 0 || l.ijk[0] >= m.n[0])) {
+                /* channel extension: the new position lies outside an open x plane,
+                 * tested every substep before the bounce gate.  The parcel leaves with
+                 * its state as advanced (no bounce); the deposit after the loop ends
+                 * its residence. */
+                vel = velNew;
+                pos = posNew;
+                exited = true;
+                break;
+            }
             /* :1306 */
             if (l.cell != posI || l.part != partI) {
                 bool hit = false;
@@ -255,6 +267,7 @@ void evolve_particle(const st_oracle_args &a, const MeshD &m, const Consts<S> &k
     }
     /* :1361 */
     for (int c = 0; c < 3; ++c) src[3 * posI + c] = src[3 * posI + c] + mByRho * (s0[c] - widen(velNew.c[c]));
+    return exited;
 }
 
 /* :1362 bounding box of the positions a call ends with (after all its passes) */
@@ -295,6 +308,7 @@ void run_parcel(st_oracle_args *a, const MeshD &m, const Consts<S> &k, int64_t i
     S *X = (S *)a->x, *Y = (S *)a->y, *Z = (S *)a->z;
     S *U = (S *)a->u, *V = (S *)a->v, *W = (S *)a->w;
     const S dia = d[i];
+    if (!(dia > (S)0)) return; /* removed by an open boundary earlier: never touched again */
     S rho = k.rho_chunk;
     double rdr = a->rho_d_by_rho_c;
     if (a->rho) {
@@ -303,12 +317,16 @@ void run_parcel(st_oracle_args *a, const MeshD &m, const Consts<S> &k, int64_t i
     }
     Vec3<S> pos{{X[i], Y[i], Z[i]}}, vel{{U[i], V[i], W[i]}};
     const int per = a->substeps_per_pass > 0 ? a->substeps_per_pass : a->nsub;
-    for (int done = 0; done < a->nsub; done += per) {
+    bool exited = false;
+    for (int done = 0; done < a->nsub && !exited; done += per) {
         const int ns = std::min(per, a->nsub - done);
         int64_t *ev = a->events ? a->events + i * (int64_t)a->nsub + done : nullptr;
-        evolve_particle<S>(*a, m, k, dia, rho, rdr, pos, vel, (const S *)a->U, ns, src, ev);
+        exited = evolve_particle<S>(*a, m, k, dia, rho, rdr, pos, vel, (const S *)a->U, ns, src, ev);
     }
-    bbox_add(bb, pos);
+    if (exited)
+        d[i] = -dia; /* marks the parcel removed; the bounding box holds parcels still in */
+    else
+        bbox_add(bb, pos);
     /* :1355-1360 */
     X[i] = pos.c[0];
     Y[i] = pos.c[1];
@@ -427,8 +445,10 @@ extern "C" int st_oracle_evolve(st_oracle_args *a) {
     for (int c = 0; c < 3; ++c)
         if (a->n[c] <= 0 || !(a->ending[c] > a->origin[c])) return -1;
     if (a->events_mode != 0 && !a->events) return -1;
+    if (a->boundary != ST_O_WALL && a->boundary != ST_O_PERIODIC && a->boundary != ST_O_CHANNEL) return -1;
+    if (a->boundary == ST_O_CHANNEL && a->events_mode != 0) return -1;
     if (a->tan_source) {
-        if (a->precision != ST_O_FP64) return -1;
+        if (a->precision != ST_O_FP64 || a->boundary == ST_O_CHANNEL) return -1;
         return evolve_dual(a);
     }
     if (a->events && a->nthreads > 1) return -1;

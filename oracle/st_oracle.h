@@ -23,6 +23,11 @@
  *                           limits, hand-derived wall/edge/corner bounces, momentum
  *                           invariant, linearity over parcels, pass-splitting telescoping,
  *                           periodic wrap cases (tests/test_oracle_*.py)
+ *   channel boundary        pinned: hand cases of exits through the outlet and the inlet,
+ *                           at a wall, on the first and last substeps, and of removed
+ *                           parcels left alone; equals the wall form for parcels that
+ *                           never reach the open planes; momentum balance with exits
+ *                           (tests/test_oracle_channel.py)
  *   st_oracle_extrapolate   pinned: closed form est_n = 2 S_{n-1} - est_{n-1}
  *   st_oracle_adjoint_*     pinned: central finite differences with frozen events and
  *                           forward-mode dual numbers of the same templated step
@@ -40,8 +45,13 @@ extern "C" {
  * `scalar` alias set to Float64).  ST_O_FP32: parcel state, fluid velocity and drag
  * in float; face test (x-o)*rDelta and the momentum source in double. */
 enum { ST_O_FP64 = 0, ST_O_FP32 = 1 };
-/* Boundary forms: wall = the reference's specular bounce; periodic = extension. */
-enum { ST_O_WALL = 0, ST_O_PERIODIC = 1 };
+/* Boundary forms: wall = the reference's specular bounce; periodic = extension;
+ * channel = extension: walls on y and z, the x planes at origin and ending open.  A
+ * substep whose new position lies outside them (tested every substep, before the bounce
+ * gate) ends the parcel: its state is kept as advanced, its residence deposit
+ * m (s0 - v') goes into the cell it was in, its diameter is stored negated, and later
+ * calls skip it (any parcel with d <= 0). */
+enum { ST_O_WALL = 0, ST_O_PERIODIC = 1, ST_O_CHANNEL = 2 };
 
 typedef struct st_oracle_args {
     int32_t precision;          /* ST_O_FP64 / ST_O_FP32 */

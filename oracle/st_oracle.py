@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libst_oracle.so")
 
 FP64, FP32 = 0, 1
-WALL, PERIODIC = 0, 1
+WALL, PERIODIC, CHANNEL = 0, 1, 2
 
 
 class Args(C.Structure):

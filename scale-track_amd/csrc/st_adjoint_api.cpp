@@ -77,6 +77,8 @@ int st_adjoint_create(st_ctx *ctx, st_cloud *c, st_field *f, int32_t nsub, int32
     st_params p;
     ST_TRY(prepare(ctx, c, f, 1.0, nsub, params, p));
     if (c->prec != ST_FP64) return fail(ST_E_UNSUPPORTED, "the adjoint runs in FP64; create the cloud with ST_FP64");
+    if (p.boundary == ST_CHANNEL)
+        return fail(ST_E_UNSUPPORTED, "the adjoint takes ST_WALL or ST_PERIODIC (removals are not differentiable)");
     if (checkpoint_every < 1 || checkpoint_every > nsub)
         return fail(ST_E_INVALID, "checkpoint_every must be in [1, nsub] (got %d)", checkpoint_every);
     st_adjoint *a = new (std::nothrow) st_adjoint;

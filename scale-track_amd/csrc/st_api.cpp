@@ -118,18 +118,18 @@ Geo make_geo(const st_mesh &m) {
 }
 
 int alloc_set(st_cloud *c, int which) {
-    const size_t bytes = (size_t)(c->n > 0 ? c->n : 1) * c->es;
+    const size_t bytes = (size_t)(c->cap > 0 ? c->cap : 1) * c->es;
     for (int f = 0; f < kNF; ++f) {
         if (f == 7 && !c->has_rho) continue;
         if (!c->buf[which][f]) ST_HIP(hipMalloc(&c->buf[which][f], bytes));
     }
-    if (!c->pid[which]) ST_HIP(hipMalloc((void **)&c->pid[which], (size_t)(c->n > 0 ? c->n : 1) * 4));
+    if (!c->pid[which]) ST_HIP(hipMalloc((void **)&c->pid[which], (size_t)(c->cap > 0 ? c->cap : 1) * 4));
     return ST_OK;
 }
 
 int validate_params(const st_params &p) {
-    if (p.boundary != ST_WALL && p.boundary != ST_PERIODIC)
-        return fail(ST_E_INVALID, "boundary must be ST_WALL or ST_PERIODIC (got %d)", p.boundary);
+    if (p.boundary != ST_WALL && p.boundary != ST_PERIODIC && p.boundary != ST_CHANNEL)
+        return fail(ST_E_INVALID, "boundary must be ST_WALL, ST_PERIODIC or ST_CHANNEL (got %d)", p.boundary);
     if (p.method != ST_METHOD_PLAIN && p.method != ST_METHOD_SORTED)
         return fail(ST_E_INVALID, "method must be ST_METHOD_PLAIN or ST_METHOD_SORTED (got %d)", p.method);
     if (p.flush != ST_FLUSH_OWNER && p.flush != ST_FLUSH_ATOMIC)
@@ -237,14 +237,14 @@ int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
                     "mesh; a sort by another brick or mesh would change what they step: destroy them first",
                     c->pins, c->sort_bd[0], c->sort_bd[1], c->sort_bd[2]);
     const int32_t rows = sort_rows(c->n, nbr);
-    const size_t n1 = (size_t)(c->n > 0 ? c->n : 1);
-    size_t bins = (size_t)c->bins_cap, bins1 = (size_t)c->bins_cap + 1, keys_cap = c->keys ? n1 : 0;
+    const size_t n1 = (size_t)(c->cap > 0 ? c->cap : 1);
+    size_t bins = (size_t)c->bins_cap, bins1 = (size_t)c->bins_cap + 1;
     size_t bins_c = bins;
     ST_TRY(grow(c, &c->counts, bins, (size_t)nbr, "sort counters"));
     ST_TRY(grow(c, &c->cursor, bins_c, (size_t)nbr, "sort cursors"));
     ST_TRY(grow(c, &c->brick_start, bins1, (size_t)nbr + 1, "brick ranges"));
     c->bins_cap = (int32_t)std::max<int64_t>(c->bins_cap, nbr);
-    ST_TRY(grow(c, &c->keys, keys_cap, n1, "sort keys"));
+    ST_TRY(grow(c, &c->keys, c->keys_cap, n1, "sort keys"));
     if (rows > 0) ST_TRY(grow(c, &c->hist, c->hist_cap, (size_t)rows * (size_t)nbr, "sort histograms"));
     ST_TRY(alloc_set(c, 1 - c->cur));
     ST_TRY(enqueue_sort<T>(c, f, p, nb, nbr, rows, nullptr, nullptr));
@@ -286,9 +286,11 @@ int range_check(st_cloud *c, st_field *f) {
     hipStream_t s = c->ctx->stream;
     ST_HIP(launch_fill_u32(c->bad, 0u, 1, s));
     if (c->prec == ST_FP64)
-        ST_HIP(launch_range_check<double>(soa<double>(c, c->cur), (uint32_t)c->n, f->geo, c->bad, s));
+        ST_HIP(launch_range_check<double>(soa<double>(c, c->cur), (uint32_t)c->fresh_from, (uint32_t)c->n, f->geo,
+                                          c->bad, s));
     else
-        ST_HIP(launch_range_check<float>(soa<float>(c, c->cur), (uint32_t)c->n, f->geo, c->bad, s));
+        ST_HIP(launch_range_check<float>(soa<float>(c, c->cur), (uint32_t)c->fresh_from, (uint32_t)c->n, f->geo,
+                                         c->bad, s));
     unsigned int bad = 0;
     ST_HIP(hipMemcpyAsync(&bad, c->bad, 4, hipMemcpyDeviceToHost, s));
     ST_HIP(hipStreamSynchronize(s));
@@ -298,6 +300,7 @@ int range_check(st_cloud *c, st_field *f) {
                     "floor((x - origin) / delta) is outside [0, N) on some axis",
                     bad);
     c->fresh = false;
+    c->fresh_from = c->n;
     return ST_OK;
 }
 
@@ -308,7 +311,7 @@ int range_check(st_cloud *c, st_field *f) {
 template <class T>
 static int multi_grid(const st_ctx *ctx, const st_cloud *c, const st_params &p, int32_t tile_cells) {
     const size_t lds = (size_t)tile_cells * 3 * (sizeof(double) + sizeof(T));
-    const int per_cu = pass_multi_blocks_per_cu<T>(p.boundary == ST_PERIODIC, c->has_rho, p.block, lds);
+    const int per_cu = pass_multi_blocks_per_cu<T>(p.boundary, c->has_rho, p.block, lds);
     if (per_cu < 1) return 0;
     for (int m = per_cu; m >= 1; --m) {
         const int64_t g = (int64_t)m * ctx->cus;
@@ -353,7 +356,7 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     a.flush = p.flush;
     a.np = (uint32_t)c->n;
     const int per = p.substeps_per_pass > 0 ? p.substeps_per_pass : nsub;
-    const int periodic = p.boundary == ST_PERIODIC;
+    const int bnd = p.boundary;
     // passes of one persistent launch (sorted method, no checkpoint)
     const int group = p.method == ST_METHOD_SORTED && !ck ? std::max(1, p.passes_per_launch) : 1;
     const int grid = group > 1 ? multi_grid<T>(ctx, c, p, a.tile_cells) : 0;
@@ -376,11 +379,11 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
         a.part = next >= nsub ? sb.part : nullptr;
         a.ck = done == 0 ? ck : nullptr;
         if (k > 1)
-            ST_HIP(launch_pass_multi<T>(a, k, periodic, p.block, grid, s));
+            ST_HIP(launch_pass_multi<T>(a, k, bnd, p.block, grid, s));
         else if (p.method == ST_METHOD_SORTED)
-            ST_HIP(launch_pass_sorted<T>(a, periodic, p.block, s));
+            ST_HIP(launch_pass_sorted<T>(a, bnd, p.block, s));
         else
-            ST_HIP(launch_pass_plain<T>(a, periodic, p.block, s));
+            ST_HIP(launch_pass_plain<T>(a, bnd, p.block, s));
         passes += k;
         done = next;
     }
@@ -602,6 +605,7 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
     c->prec = precision;
     c->es = precision == ST_FP64 ? 8 : 4;
     c->n = n;
+    c->cap = n;
     c->mu_c = mu_c;
     c->rho_d = per_parcel_rho ? 1.0 : rho_d;
     c->rdr = per_parcel_rho ? 0.0 : rho_d_by_rho_c;
@@ -680,6 +684,7 @@ int st_cloud_upload(st_cloud *c, const void *d, const void *x, const void *y, co
     ST_HIP(hipStreamSynchronize(s));
     c->uploaded = true;
     c->fresh = true;
+    c->fresh_from = 0;
     c->permuted = false;
     c->sorted = false;
     c->strays_pending = false;
@@ -708,6 +713,150 @@ int st_cloud_download(st_cloud *c, void *d, void *x, void *y, void *z, void *u, 
     for (int f = 0; f < 7; ++f)
         if (bytes) ST_HIP(hipMemcpyAsync(dst[f], c->buf[from][f], bytes, hipMemcpyDeviceToHost, s));
     ST_HIP(hipStreamSynchronize(s));
+    return ST_OK;
+}
+
+int st_cloud_reserve(st_cloud *c, int64_t capacity) {
+    if (!c) return fail(ST_E_INVALID, "null cloud");
+    ST_TRY(check_ctx(c->ctx));
+    if (capacity <= c->cap) return ST_OK;
+    if (capacity > 0x7fffffffLL || (uint64_t)capacity * c->es >= (1ull << 32))
+        return fail(ST_E_RANGE, "a capacity of %lld parcels exceeds the 4 GiB per-array limit", (long long)capacity);
+    if (c->pins > 0)
+        return fail(ST_E_STATE, "cloud is used by %d live graph(s), batch(es) or adjoint(s); its arrays cannot move",
+                    c->pins);
+    hipStream_t s = c->ctx->stream;
+    ST_HIP(hipStreamSynchronize(s));
+    // the current set moves into arrays of the new capacity; the other set and the sort
+    // keys are dropped and come back at the new size when a sort needs them
+    void *nb[kNF] = {};
+    int32_t *npid = nullptr;
+    int r = ST_OK;
+    for (int f = 0; f < kNF && r == ST_OK; ++f) {
+        if (f == 7 && !c->has_rho) continue;
+        if (hipMalloc(&nb[f], (size_t)capacity * c->es) != hipSuccess) r = ST_E_NOMEM;
+    }
+    if (r == ST_OK && hipMalloc((void **)&npid, (size_t)capacity * 4) != hipSuccess) r = ST_E_NOMEM;
+    if (r != ST_OK) {
+        for (void *q : nb)
+            if (q) (void)hipFree(q);
+        if (npid) (void)hipFree(npid);
+        return fail(ST_E_NOMEM, "cannot grow the cloud to %lld parcels", (long long)capacity);
+    }
+    for (int f = 0; f < kNF; ++f) {
+        if (f == 7 && !c->has_rho) continue;
+        if (c->n) ST_HIP(hipMemcpyAsync(nb[f], c->buf[c->cur][f], (size_t)c->n * c->es, hipMemcpyDeviceToDevice, s));
+    }
+    if (c->n) ST_HIP(hipMemcpyAsync(npid, c->pid[c->cur], (size_t)c->n * 4, hipMemcpyDeviceToDevice, s));
+    ST_HIP(hipStreamSynchronize(s));
+    for (int w = 0; w < 2; ++w) {
+        for (int f = 0; f < kNF; ++f)
+            if (c->buf[w][f]) ST_HIP(hipFree(c->buf[w][f]));
+        if (c->pid[w]) ST_HIP(hipFree(c->pid[w]));
+        for (int f = 0; f < kNF; ++f) c->buf[w][f] = nullptr;
+        c->pid[w] = nullptr;
+    }
+    for (int f = 0; f < kNF; ++f) c->buf[0][f] = nb[f];
+    c->pid[0] = npid;
+    c->cur = 0;
+    if (c->keys) ST_HIP(hipFree(c->keys));
+    c->keys = nullptr;
+    c->keys_cap = 0;
+    c->cap = capacity;
+    ++c->epoch;
+    return ST_OK;
+}
+
+int st_cloud_inject(st_cloud *c, int64_t k, const void *d, const void *x, const void *y, const void *z,
+                    const void *u, const void *v, const void *w, const void *rho) {
+    if (!c) return fail(ST_E_INVALID, "null cloud");
+    ST_TRY(check_ctx(c->ctx));
+    if (k < 0) return fail(ST_E_INVALID, "negative parcel count");
+    if (k == 0) return ST_OK;
+    if (c->pins > 0)
+        return fail(ST_E_STATE,
+                    "cloud is used by %d live graph(s), batch(es) or adjoint(s), which step a fixed parcel count",
+                    c->pins);
+    if (!c->uploaded && c->n > 0) return fail(ST_E_STATE, "upload the cloud's parcels before injecting more");
+    const void *src[kNF] = {d, x, y, z, u, v, w, rho};
+    for (int f = 0; f < 7; ++f)
+        if (!src[f]) return fail(ST_E_INVALID, "null parcel array %d", f);
+    if (c->has_rho && !rho) return fail(ST_E_INVALID, "cloud has per-parcel density but rho is NULL");
+    for (int f = 0; f < kNF; ++f) {
+        if (f == 7 && !c->has_rho) continue;
+        if (!finite_all(src[f], k, c->es)) return fail(ST_E_INVALID, "injected array %d holds a non-finite value", f);
+    }
+    if (!positive_all(d, k, c->es)) return fail(ST_E_INVALID, "injected diameters must be > 0");
+    if (c->has_rho && !positive_all(rho, k, c->es)) return fail(ST_E_INVALID, "injected densities must be > 0");
+    const int64_t n1 = c->n + k;
+    if (n1 > c->cap) ST_TRY(st_cloud_reserve(c, std::max<int64_t>(n1, c->cap + c->cap / 2)));
+    hipStream_t s = c->ctx->stream;
+    for (int f = 0; f < kNF; ++f) {
+        if (f == 7 && !c->has_rho) continue;
+        ST_HIP(hipMemcpyAsync((char *)c->buf[c->cur][f] + (size_t)c->n * c->es, src[f], (size_t)k * c->es,
+                              hipMemcpyHostToDevice, s));
+    }
+    ST_HIP(st::launch_iota_from(c->pid[c->cur] + c->n, (uint32_t)k, (int32_t)c->n, s));
+    ST_HIP(hipStreamSynchronize(s));
+    if (!c->fresh) c->fresh_from = c->n;
+    c->fresh = true;
+    c->n = n1;
+    c->uploaded = true;
+    c->sorted = false;  // the new parcels sit at the end; the next sorted step sorts
+    ++c->epoch;
+    return ST_OK;
+}
+
+int st_cloud_compact(st_cloud *c, int64_t *n_live) {
+    if (!c) return fail(ST_E_INVALID, "null cloud");
+    ST_TRY(check_ctx(c->ctx));
+    if (c->pins > 0)
+        return fail(ST_E_STATE,
+                    "cloud is used by %d live graph(s), batch(es) or adjoint(s), which step a fixed parcel count",
+                    c->pins);
+    if (!c->uploaded || c->n == 0) {
+        if (n_live) *n_live = c->n;
+        return ST_OK;
+    }
+    hipStream_t s = c->ctx->stream;
+    const uint32_t np = (uint32_t)c->n;
+    const uint32_t tiles = st::scan_tiles(np);
+    ST_TRY(alloc_set(c, 1 - c->cur));
+    uint32_t *tmp = nullptr;
+    const size_t words = 4 * (size_t)np + 4 * ((size_t)tiles + 1);
+    ST_HIP(hipMalloc((void **)&tmp, words * 4));
+    uint32_t *live_dev = tmp, *live_up = tmp + np, *pos_dev = tmp + 2 * (size_t)np, *new_up = tmp + 3 * (size_t)np;
+    uint32_t *tile_sums = tmp + 4 * (size_t)np;
+    int32_t *off_dev = (int32_t *)(tile_sums + tiles + 1);
+    int32_t *off_up = off_dev + tiles + 1;
+    uint32_t *cursor = (uint32_t *)(off_up + tiles + 1);
+    hipError_t e = c->prec == ST_FP64
+                       ? st::launch_compact<double>(soa<double>(c, c->cur), c->pid[c->cur], soa<double>(c, 1 - c->cur),
+                                                    c->pid[1 - c->cur], np, live_dev, live_up, pos_dev, new_up,
+                                                    tile_sums, off_dev, off_up, cursor, s)
+                       : st::launch_compact<float>(soa<float>(c, c->cur), c->pid[c->cur], soa<float>(c, 1 - c->cur),
+                                                   c->pid[1 - c->cur], np, live_dev, live_up, pos_dev, new_up,
+                                                   tile_sums, off_dev, off_up, cursor, s);
+    int32_t live = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&live, off_dev + tiles, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && live > 0) {
+        if (c->prec == ST_FP64)
+            e = st::launch_copy_back<double>(soa<double>(c, 1 - c->cur), c->pid[1 - c->cur], soa<double>(c, c->cur),
+                                             c->pid[c->cur], (uint32_t)live, nullptr, nullptr, s);
+        else
+            e = st::launch_copy_back<float>(soa<float>(c, 1 - c->cur), c->pid[1 - c->cur], soa<float>(c, c->cur),
+                                            c->pid[c->cur], (uint32_t)live, nullptr, nullptr, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(ST_E_DEVICE, "compaction: %s", hipGetErrorString(e));
+    c->n = live;
+    c->permuted = true;
+    c->sorted = false;
+    c->fresh_from = c->fresh ? 0 : c->n;  // unchecked parcels moved: check them all
+    ++c->epoch;
+    if (n_live) *n_live = live;
     return ST_OK;
 }
 

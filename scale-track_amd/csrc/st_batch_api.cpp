@@ -220,6 +220,7 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
         st_params_default(&p);
     p.method = ST_METHOD_SORTED;
     ST_TRY(validate_params(p));
+    if (p.boundary == ST_CHANNEL) return fail(ST_E_UNSUPPORTED, "batches take ST_WALL or ST_PERIODIC clouds");
     std::vector<int> idx(n);
     std::vector<double> density(n);
     for (int i = 0; i < n; ++i) {

@@ -10,6 +10,14 @@ namespace st {
 
 constexpr int kWave = 64;
 
+// Boundary forms of the kernels (the st_boundary values): the reference's walls; the
+// periodic extension; and the channel extension -- walls on y and z, while the x planes
+// at origin and ending are open: a parcel whose new position lies outside them leaves
+// the domain (kExit below) and is marked removed by a negative diameter.
+constexpr int kWall = 0, kPeriodic = 1, kChannel = 2;
+// Bit of advance()'s result: the substep carried the parcel out through an open plane.
+constexpr unsigned kExit = 8u;
+
 __device__ __forceinline__ int32_t axis_floor(double x, double o, double rd) {
     return (int32_t)floor((x - o) * rd);
 }
@@ -204,8 +212,11 @@ struct Quot<float> {
 // drag update and advance, relocation, then the specular wall bounce -- only when the
 // folded cell or partition changed -- or the periodic wrap.  On return (n*) is the new
 // state, nc the cell after relocation, and the result has bit a set for each axis a
-// that bounced (the caller applies the source correction s0_a -= 2 v_a of :734).
-template <class T, bool PERIODIC>
+// that bounced (the caller applies the source correction s0_a -= 2 v_a of :734).  In
+// the channel form a new position outside the open x planes (tested on every substep,
+// before the reference's bounce gate) returns kExit alone: no bounce, the state as
+// advanced.
+template <class T, int BND>
 __device__ __forceinline__ unsigned advance(const Geo &g, const Phys<T> &ph, T f, const Quot<T> &den, T ux, T uy,
                                             T uz, T px, T py, T pz, T vx, T vy, T vz, const Cell &cell,
                                             int32_t part0, T &nx, T &ny, T &nz, T &nvx, T &nvy, T &nvz,
@@ -217,8 +228,9 @@ __device__ __forceinline__ unsigned advance(const Geo &g, const Phys<T> &ph, T f
     nz = pz + nvz * ph.dt;
     int32_t I[3];
     unsigned bounced = 0;
-    if (!PERIODIC) {
+    if (BND != kPeriodic) {
         locate<T, false>(g, nx, ny, nz, I, nc);  // :1303
+        if (BND == kChannel && (uint32_t)I[0] >= (uint32_t)g.n[0]) return kExit;
         if (!same_cell(nc, cell) || nc.part != part0) {  // :1306
             if (I[0] < 0 || I[0] >= g.n[0]) {  // :1309-1329
                 nx = px;

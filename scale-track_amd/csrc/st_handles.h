@@ -57,6 +57,7 @@ struct st_cloud {
     st_ctx *ctx = nullptr;
     int prec = ST_FP64;
     int64_t n = 0;
+    int64_t cap = 0;  // parcels each device array holds (n <= cap; grown by inject/reserve)
     size_t es = 8;
     double mu_c = 0, rho_d = 0, rdr = 0, rho_c = 0;
     int has_rho = 0;
@@ -64,10 +65,12 @@ struct st_cloud {
     int32_t *pid[2] = {nullptr, nullptr};
     int cur = 0;
     bool uploaded = false;
-    bool fresh = false;     // uploaded since the last range check
+    bool fresh = false;     // parcels from fresh_from on were not yet range-checked
+    int64_t fresh_from = 0;
     bool permuted = false;  // device order differs from upload order
     // sort state
     uint32_t *keys = nullptr;
+    size_t keys_cap = 0;
     uint32_t *counts = nullptr;
     uint32_t *cursor = nullptr;
     uint32_t *hist = nullptr;            // per-chunk brick histograms of the chunked sort

@@ -104,16 +104,17 @@ inline size_t tile_lds_bytes(const int32_t bd[3]) {
 
 // Host-side launchers (st_step.hip).
 namespace st {
+// bnd: the st_boundary form (wall 0, periodic 1, channel 2).
 template <class T>
-hipError_t launch_pass_sorted(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
+hipError_t launch_pass_sorted(const PassArgs<T> &a, int bnd, int block, hipStream_t s);
 template <class T>
-hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s);
+hipError_t launch_pass_plain(const PassArgs<T> &a, int bnd, int block, hipStream_t s);
 // npass passes of a.nsteps substeps each in one persistent launch of `grid` workgroups
 // (each owning the same bricks in every pass); a.part is filled in the last pass.
 template <class T>
-hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, int periodic, int block, int grid, hipStream_t s);
+hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, int bnd, int block, int grid, hipStream_t s);
 template <class T>
-int pass_multi_blocks_per_cu(int periodic, int rho, int block, size_t lds);
+int pass_multi_blocks_per_cu(int bnd, int rho, int block, size_t lds);
 // Persistent pass over clouds [0, ncloud) of descs: work items item0[0] .. item0[ncloud]-1
 // (counter must hold item0[0] at launch); rho: every one of these clouds carries a
 // per-parcel density array (else none does).
@@ -155,6 +156,19 @@ hipError_t launch_unpermute(const Soa<T> &in, const int32_t *pid, Soa<T> out, ui
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s, const unsigned int *gate = nullptr);
 hipError_t launch_fill_f64(double *p, double value, size_t n, hipStream_t s);
 hipError_t launch_iota(int32_t *p, uint32_t n, hipStream_t s);
+hipError_t launch_iota_from(int32_t *p, uint32_t n, int32_t base, hipStream_t s);
+// Tiles of the exclusive scan over n elements (entries of its tile-sum array).
+uint32_t scan_tiles(uint32_t n);
+// Drops removed parcels (d <= 0) of `in` into `out`, keeping the device order of the
+// rest and renumbering their upload-order indices among themselves.  offsets_dev
+// (scan_tiles(np) + 1 entries) ends with the number of parcels kept.  Scratch:
+// live_dev, live_up, pos_dev, new_up (np each), tile_sums (scan_tiles(np)),
+// offsets_up (scan_tiles(np) + 1), cursor (scan_tiles(np)).
+template <class T>
+hipError_t launch_compact(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out, uint32_t np,
+                          uint32_t *live_dev, uint32_t *live_up, uint32_t *pos_dev, uint32_t *new_up,
+                          uint32_t *tile_sums, int32_t *offsets_dev, int32_t *offsets_up, uint32_t *cursor,
+                          hipStream_t s);
 hipError_t launch_merge(double *src, double *stray, size_t n, hipStream_t s);
 hipError_t launch_copy(const void *a, void *b, size_t bytes, hipStream_t s);
 
@@ -180,6 +194,6 @@ hipError_t launch_gather_pid(const double *src, const int32_t *pid, double *dst,
                              hipStream_t s);
 hipError_t launch_scatter_pid(const double *src, const int32_t *pid, double *dst, uint32_t n, hipStream_t s);
 template <class T>
-hipError_t launch_range_check(const Soa<T> &s, uint32_t np, Geo g, unsigned int *bad,
+hipError_t launch_range_check(const Soa<T> &s, uint32_t p0, uint32_t np, Geo g, unsigned int *bad,
                               hipStream_t st);
 }  // namespace st

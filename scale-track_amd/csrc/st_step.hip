@@ -16,7 +16,26 @@
 //    HBM gathers and FP64 atomics into a separate array.
 #include "st_device.h"
 
+#include <type_traits>
+
 namespace st {
+
+// Calls f(std::integral_constant<int, BND>, std::bool_constant<RHO>) for the runtime
+// boundary form (kWall, kPeriodic, kChannel) and density form: the launchers pick the
+// kernel instance with it.
+template <class F>
+inline void with_forms(int bnd, bool rho, F &&f) {
+    using W = std::integral_constant<int, kWall>;
+    using P = std::integral_constant<int, kPeriodic>;
+    using C = std::integral_constant<int, kChannel>;
+    if (bnd == kPeriodic) {
+        if (rho) f(P{}, std::true_type{}); else f(P{}, std::false_type{});
+    } else if (bnd == kChannel) {
+        if (rho) f(C{}, std::true_type{}); else f(C{}, std::false_type{});
+    } else {
+        if (rho) f(W{}, std::true_type{}); else f(W{}, std::false_type{});
+    }
+}
 
 // Everything in HBM: the plain method.
 //
@@ -149,10 +168,14 @@ struct NoPrefetch {
 // once the parcel's starting cell velocity has been read, so loads it issues (the next
 // parcel's state) stay in flight across the substeps: vector-memory loads complete in
 // order, and a later HBM gather would otherwise wait for them.
-template <class T, bool PERIODIC, bool RHO, class Env, class Prefetch>
+template <class T, int BND, bool RHO, class Env, class Prefetch>
 __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
                                             const ParcelIn<T> &in, uint32_t p, int32_t nsteps,
                                             Env &env, Box<T> &box, const Prefetch &prefetch) {
+    if (BND == kChannel && !(in.d > (T)0)) {  // removed earlier: never touched again
+        prefetch();
+        return;
+    }
     double m;
     T f, den;
     parcel_consts(ph, RHO, in.d, in.rho, m, f, den);
@@ -163,7 +186,7 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
 
     int32_t I[3];
     Cell cell;
-    locate<T, PERIODIC>(g, px, py, pz, I, cell);  // :1286
+    locate<T, BND == kPeriodic>(g, px, py, pz, I, cell);  // :1286
     const int32_t part0 = cell.part;
     env.enter(cell);
     typename Env::Slot sl = env.slot(cell);
@@ -172,11 +195,22 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     double s0x = (double)vx, s0y = (double)vy, s0z = (double)vz;  // :1289
     prefetch();
 
+    bool exited = false;
     for (int32_t t = 1; t <= nsteps; ++t) {
         T nx, ny, nz, nvx, nvy, nvz;
         Cell nc;
-        const unsigned b = advance<T, PERIODIC>(g, ph, f, q, ux, uy, uz, px, py, pz, vx, vy, vz, cell,
-                                                part0, nx, ny, nz, nvx, nvy, nvz, nc);
+        const unsigned b = advance<T, BND>(g, ph, f, q, ux, uy, uz, px, py, pz, vx, vy, vz, cell,
+                                           part0, nx, ny, nz, nvx, nvy, nvz, nc);
+        if (BND == kChannel && (b & kExit)) {  // the residence, and the parcel, end here
+            vx = nvx;
+            vy = nvy;
+            vz = nvz;
+            px = nx;
+            py = ny;
+            pz = nz;
+            exited = true;
+            break;
+        }
         if (b & 1u) s0x = s0x - 2.0 * (double)vx;  // update_at_boundary!, :730-736
         if (b & 2u) s0y = s0y - 2.0 * (double)vy;
         if (b & 4u) s0z = s0z - 2.0 * (double)vz;
@@ -205,7 +239,10 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     st_off(s.v, off, vy);
     st_off(s.w, off, vz);
     env.deposit(cell, sl, m * (s0x - (double)vx), m * (s0y - (double)vy), m * (s0z - (double)vz));
-    box.add(px, py, pz);  // :1362
+    if (BND == kChannel && exited)
+        st_off(s.d, off, -in.d);
+    else
+        box.add(px, py, pz);  // :1362
 }
 
 // The same step for a pass of exactly one substep (the schedule of one launch per
@@ -213,10 +250,14 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
 // inside the loop (:1342) and never uses the new cell, so the relocation after a bounce
 // or wrap (:1333-1335) has no effect on any output and is left out; every value that
 // is written is computed by the same operations as in step_parcel.
-template <class T, bool PERIODIC, bool RHO, class Env, class Prefetch>
+template <class T, int BND, bool RHO, class Env, class Prefetch>
 __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
                                                 const ParcelIn<T> &in, uint32_t p, Env &env, Box<T> &box,
                                                 bool stats, const Prefetch &prefetch) {
+    if (BND == kChannel && !(in.d > (T)0)) {  // removed earlier: never touched again
+        prefetch();
+        return;
+    }
     double m;
     T f, den;
     parcel_consts(ph, RHO, in.d, in.rho, m, f, den);
@@ -225,7 +266,7 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
 
     int32_t I[3];
     Cell cell;
-    locate<T, PERIODIC>(g, px, py, pz, I, cell);  // :1286
+    locate<T, BND == kPeriodic>(g, px, py, pz, I, cell);  // :1286
     env.enter(cell);
     const typename Env::Slot sl = env.slot(cell);
     T ux, uy, uz;
@@ -240,10 +281,13 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
     T ny = py + nvy * ph.dt;
     T nz = pz + nvz * ph.dt;
     double s0x = (double)vx, s0y = (double)vy, s0z = (double)vz;  // :1289
-    if (!PERIODIC) {
+    bool exited = false;
+    if (BND != kPeriodic) {
         Cell nc;
         locate<T, false>(g, nx, ny, nz, I, nc);  // :1303
-        if (!same_cell(nc, cell) || nc.part != cell.part) {  // :1306
+        if (BND == kChannel && (uint32_t)I[0] >= (uint32_t)g.n[0]) {
+            exited = true;  // out through an open plane: no bounce, the state as advanced
+        } else if (!same_cell(nc, cell) || nc.part != cell.part) {  // :1306
             if (I[0] < 0 || I[0] >= g.n[0]) {  // :1309-1329, :730-736
                 nx = px;
                 nvx = -vx;
@@ -281,7 +325,10 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
     st_off(s.v, off, nvy);
     st_off(s.w, off, nvz);
     env.deposit(cell, sl, m * (s0x - (double)nvx), m * (s0y - (double)nvy), m * (s0z - (double)nvz));
-    if (stats) box.add(nx, ny, nz);  // :1362
+    if (BND == kChannel && exited)
+        st_off(s.d, off, -in.d);
+    else if (stats)
+        box.add(nx, ny, nz);  // :1362
 }
 
 // Workgroup reduction of the box and the stray count into this workgroup's statistics
@@ -319,7 +366,7 @@ __device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, double
     }
 }
 
-template <class T, bool PERIODIC, bool RHO>
+template <class T, int BND, bool RHO>
 __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     Box<T> box;
@@ -329,9 +376,9 @@ __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
         ParcelIn<T> in;
         load_parcel<T, RHO>(a.s, p, a.ph.rho, in);
         if (a.nsteps == 1)
-            step_parcel_one<T, PERIODIC, RHO>(a.g, a.ph, a.s, in, p, env, box, true, NoPrefetch{});
+            step_parcel_one<T, BND, RHO>(a.g, a.ph, a.s, in, p, env, box, true, NoPrefetch{});
         else
-            step_parcel<T, PERIODIC, RHO>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
+            step_parcel<T, BND, RHO>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
     }
     if (a.part) flush_stats(box, 0u, a.part + (size_t)blockIdx.x * kStatSlot);
 }
@@ -371,7 +418,7 @@ struct TileShape {
 
 // One brick of a sorted cloud: stage its velocities in LDS, step its parcels, add its
 // source sums to HBM once per cell.  Every thread of the workgroup must call it.
-template <class T, bool PERIODIC, bool RHO, bool CKPT = false>
+template <class T, int BND, bool RHO, bool CKPT = false>
 __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t brick, int32_t nsteps,
                                               double *lds, double *part) {
     const int32_t bi = brick % a.nb[0];
@@ -496,7 +543,7 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             auto one = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
                 const uint32_t q = ahead(p);
                 if (CKPT) save_checkpoint(a.ck, a.np, (uint32_t)p, cur);
-                step_parcel_one<T, PERIODIC, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, env, box, stats,
+                step_parcel_one<T, BND, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, env, box, stats,
                                                   [&] { load_parcel<T, RHO>(a.s, q, a.ph.rho, fill); });
                 p += bs;
             };
@@ -510,7 +557,7 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             auto many = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
                 const uint32_t q = ahead(p);
                 if (CKPT) save_checkpoint(a.ck, a.np, (uint32_t)p, cur);
-                step_parcel<T, PERIODIC, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, nsteps, env, box,
+                step_parcel<T, BND, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, nsteps, env, box,
                                               [&] { load_parcel<T, RHO>(a.s, q, a.ph.rho, fill); });
                 p += bs;
             };
@@ -578,14 +625,14 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     if (part) flush_stats(box, env.strays, part + (size_t)brick * kStatSlot);
 }
 
-template <class T, bool PERIODIC, bool RHO, bool CKPT = false>
+template <class T, int BND, bool RHO, bool CKPT = false>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4))) k_pass_sorted(PassArgs<T> a) {
     extern __shared__ double lds[];
     // Bricks in dispatch order: workgroups go round-robin over the XCDs, so the bricks
     // in flight at any time form one contiguous window of the parcel arrays (measured
     // faster on MI355X than giving each XCD its own contiguous range of bricks, which
     // spreads the concurrent streams over eight windows).
-    process_brick<T, PERIODIC, RHO, CKPT>(a, blockIdx.x, a.nsteps, lds, a.part);
+    process_brick<T, BND, RHO, CKPT>(a, blockIdx.x, a.nsteps, lds, a.part);
 }
 
 // Several passes of one sorted cloud in one persistent launch.  Workgroup w owns bricks
@@ -595,48 +642,38 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeo
 // its barriers ordering them, and everything shared (halo and stray sums) goes to
 // `stray` with atomics.  Pass-major order keeps each pass a full sweep over the
 // parcel arrays in HBM.  `part` (statistics) is filled in the last pass only.
-template <class T, bool PERIODIC, bool RHO>
+template <class T, int BND, bool RHO>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4)))
 k_pass_multi(PassArgs<T> a, int32_t npass) {
     extern __shared__ double lds[];
     for (int32_t k = 0; k < npass; ++k) {
         double *part = k == npass - 1 ? a.part : nullptr;
         for (int32_t brick = blockIdx.x; brick < a.nbricks; brick += gridDim.x) {
-            process_brick<T, PERIODIC, RHO>(a, brick, a.nsteps, lds, part);
+            process_brick<T, BND, RHO>(a, brick, a.nsteps, lds, part);
             __syncthreads();  // the flush's tile reads are done before the next staging
         }
     }
 }
 
 template <class T>
-hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, int periodic, int block, int grid, hipStream_t s) {
+hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, int bnd, int block, int grid, hipStream_t s) {
     const size_t lds = (size_t)a.tile_cells * 3 * (sizeof(double) + sizeof(T));
-    const bool rho = a.s.rho != nullptr;
-    if (periodic && rho)
-        k_pass_multi<T, true, true><<<grid, block, lds, s>>>(a, npass);
-    else if (periodic)
-        k_pass_multi<T, true, false><<<grid, block, lds, s>>>(a, npass);
-    else if (rho)
-        k_pass_multi<T, false, true><<<grid, block, lds, s>>>(a, npass);
-    else
-        k_pass_multi<T, false, false><<<grid, block, lds, s>>>(a, npass);
+    with_forms(bnd, a.s.rho != nullptr, [&](auto B, auto R) {
+        k_pass_multi<T, decltype(B)::value, decltype(R)::value><<<grid, block, lds, s>>>(a, npass);
+    });
     return hipGetLastError();
 }
 template hipError_t launch_pass_multi<float>(const PassArgs<float> &, int32_t, int, int, int, hipStream_t);
 template hipError_t launch_pass_multi<double>(const PassArgs<double> &, int32_t, int, int, int, hipStream_t);
 
 template <class T>
-int pass_multi_blocks_per_cu(int periodic, int rho, int block, size_t lds) {
+int pass_multi_blocks_per_cu(int bnd, int rho, int block, size_t lds) {
     int n = 0;
-    hipError_t e;
-    if (periodic && rho)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_multi<T, true, true>, block, lds);
-    else if (periodic)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_multi<T, true, false>, block, lds);
-    else if (rho)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_multi<T, false, true>, block, lds);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_multi<T, false, false>, block, lds);
+    hipError_t e = hipSuccess;
+    with_forms(bnd, rho != 0, [&](auto B, auto R) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_multi<T, decltype(B)::value, decltype(R)::value>,
+                                                         block, lds);
+    });
     return e == hipSuccess ? n : 0;
 }
 template int pass_multi_blocks_per_cu<float>(int, int, int, size_t);
@@ -653,7 +690,7 @@ template int pass_multi_blocks_per_cu<double>(int, int, int, size_t);
 // between items.  The FP32 instances are held to 80 VGPRs (6 waves per SIMD), the
 // budget of the single-cloud kernel: the loop over items otherwise keeps enough
 // per-thread values live across items to drop the occupancy by a wave.
-template <class T, bool PERIODIC, bool RHO>
+template <class T, int BND, bool RHO>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4)))
 k_pass_batch(const PassArgs<T> *__restrict__ descs, const int32_t *__restrict__ item0, int32_t ncloud,
              int32_t nsteps, int stats, unsigned int *counter) {
@@ -676,7 +713,7 @@ k_pass_batch(const PassArgs<T> *__restrict__ descs, const int32_t *__restrict__ 
         }
         const int32_t c = __builtin_amdgcn_readfirstlane(lo);
         const PassArgs<T> a = descs[c];
-        process_brick<T, PERIODIC, RHO>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
+        process_brick<T, BND, RHO>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
         __syncthreads();  // the flush's tile reads are done before the next staging
         if (threadIdx.x == 0) s_item = next;
         __syncthreads();
@@ -721,46 +758,29 @@ template int batch_max_blocks_per_cu<float>(int, int, int, size_t);
 template int batch_max_blocks_per_cu<double>(int, int, int, size_t);
 
 template <class T>
-hipError_t launch_pass_plain(const PassArgs<T> &a, int periodic, int block, hipStream_t s) {
+hipError_t launch_pass_plain(const PassArgs<T> &a, int bnd, int block, hipStream_t s) {
     if (a.np == 0) return hipSuccess;
     const uint32_t grid = (a.np + block - 1) / block;
-    const bool rho = a.s.rho != nullptr;
-    if (periodic && rho)
-        k_pass_plain<T, true, true><<<grid, block, 0, s>>>(a);
-    else if (periodic)
-        k_pass_plain<T, true, false><<<grid, block, 0, s>>>(a);
-    else if (rho)
-        k_pass_plain<T, false, true><<<grid, block, 0, s>>>(a);
-    else
-        k_pass_plain<T, false, false><<<grid, block, 0, s>>>(a);
+    with_forms(bnd, a.s.rho != nullptr, [&](auto B, auto R) {
+        k_pass_plain<T, decltype(B)::value, decltype(R)::value><<<grid, block, 0, s>>>(a);
+    });
     return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_pass_sorted(const PassArgs<T> &a, int periodic, int block, hipStream_t s) {
+hipError_t launch_pass_sorted(const PassArgs<T> &a, int bnd, int block, hipStream_t s) {
     const size_t lds = (size_t)a.tile_cells * 3 * (sizeof(double) + sizeof(T));
-    const bool rho = a.s.rho != nullptr;
     if constexpr (sizeof(T) == 8) {
-        if (a.ck) {
-            if (periodic && rho)
-                k_pass_sorted<T, true, true, true><<<a.nbricks, block, lds, s>>>(a);
-            else if (periodic)
-                k_pass_sorted<T, true, false, true><<<a.nbricks, block, lds, s>>>(a);
-            else if (rho)
-                k_pass_sorted<T, false, true, true><<<a.nbricks, block, lds, s>>>(a);
-            else
-                k_pass_sorted<T, false, false, true><<<a.nbricks, block, lds, s>>>(a);
+        if (a.ck) {  // the adjoint's forward: wall and periodic forms only
+            with_forms(bnd == kPeriodic ? kPeriodic : kWall, a.s.rho != nullptr, [&](auto B, auto R) {
+                k_pass_sorted<T, decltype(B)::value, decltype(R)::value, true><<<a.nbricks, block, lds, s>>>(a);
+            });
             return hipGetLastError();
         }
     }
-    if (periodic && rho)
-        k_pass_sorted<T, true, true><<<a.nbricks, block, lds, s>>>(a);
-    else if (periodic)
-        k_pass_sorted<T, true, false><<<a.nbricks, block, lds, s>>>(a);
-    else if (rho)
-        k_pass_sorted<T, false, true><<<a.nbricks, block, lds, s>>>(a);
-    else
-        k_pass_sorted<T, false, false><<<a.nbricks, block, lds, s>>>(a);
+    with_forms(bnd, a.s.rho != nullptr, [&](auto B, auto R) {
+        k_pass_sorted<T, decltype(B)::value, decltype(R)::value><<<a.nbricks, block, lds, s>>>(a);
+    });
     return hipGetLastError();
 }
 
@@ -1245,11 +1265,134 @@ template hipError_t launch_unpermute<float>(const Soa<float> &, const int32_t *,
 template hipError_t launch_unpermute<double>(const Soa<double> &, const int32_t *, Soa<double>,
                                              uint32_t, hipStream_t);
 
-// Count parcels whose position is outside [origin, ending) on some axis.
+// ---------------------------------------------------------------------------------
+// Injection and compaction of a cloud.
+
+__global__ void k_iota_from(int32_t *p, uint32_t n, int32_t base) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = base + (int32_t)i;
+}
+
+hipError_t launch_iota_from(int32_t *p, uint32_t n, int32_t base, hipStream_t s) {
+    if (!n) return hipSuccess;
+    k_iota_from<<<(n + 255) / 256, 256, 0, s>>>(p, n, base);
+    return hipGetLastError();
+}
+
+// live_dev[p] = 1 for a parcel not removed (d > 0) in device order; live_up[pid[p]] the
+// same in upload order.
 template <class T>
-__global__ void k_range_check(Soa<T> s, uint32_t np, Geo g, unsigned int *bad) {
+__global__ void k_live_flags(const T *d, const int32_t *pid, uint32_t np, uint32_t *live_dev, uint32_t *live_up) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
+    const uint32_t on = d[p] > (T)0 ? 1u : 0u;
+    live_dev[p] = on;
+    live_up[(uint32_t)pid[p]] = on;
+}
+
+// Exclusive scan of n uint32 in tiles of kScanTile: tile sums, one scan of the sums
+// (k_scan), then each tile's own scan from its offset.
+constexpr int kScanPer = 16;
+constexpr int kScanTile = 256 * kScanPer;
+
+__global__ void __launch_bounds__(256) k_tile_sums(const uint32_t *in, uint32_t n, uint32_t *sums) {
+    __shared__ uint32_t red[256 / kWave];
+    const uint32_t t0 = blockIdx.x * (uint32_t)kScanTile;
+    uint32_t v = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        const uint32_t i = t0 + k * 256u + threadIdx.x;
+        if (i < n) v += in[i];
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) sums[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void __launch_bounds__(256) k_tile_scan(const uint32_t *in, uint32_t n, const int32_t *offsets,
+                                                   uint32_t *out) {
+    __shared__ uint32_t part[256];
+    const uint32_t base = blockIdx.x * (uint32_t)kScanTile + threadIdx.x * kScanPer;  // a contiguous run per thread
+    uint32_t v[kScanPer], sum = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        sum += v[k];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t add = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = (uint32_t)offsets[blockIdx.x] + part[threadIdx.x] - sum;
+    for (int k = 0; k < kScanPer; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+// out[i] = in[0] + ... + in[i-1]; offsets[t] receives each tile's start and
+// offsets[ntiles] the total (tile_sums holds ntiles = ceil(n / kScanTile) entries).
+hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *tile_sums,
+                                 int32_t *offsets, uint32_t *cursor, hipStream_t s) {
+    const uint32_t tiles = n ? (n + kScanTile - 1) / kScanTile : 1;
+    if (n) k_tile_sums<<<tiles, 256, 0, s>>>(in, n, tile_sums);
+    else k_fill_u32<<<1, 256, 0, s>>>(tile_sums, 0u, 1, nullptr);
+    k_scan<<<1, 1024, 0, s>>>(tile_sums, (int32_t)tiles, offsets, cursor, nullptr);
+    if (n) k_tile_scan<<<tiles, 256, 0, s>>>(in, n, offsets, out);
+    return hipGetLastError();
+}
+
+uint32_t scan_tiles(uint32_t n) { return n ? (n + kScanTile - 1) / kScanTile : 1; }
+
+// Live parcels moved to their place in a compacted order (device order kept), with
+// their upload-order index renumbered among the live ones.
+template <class T>
+__global__ void k_compact_scatter(Soa<T> in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out,
+                                  const uint32_t *pos_dev, const uint32_t *new_up, uint32_t np) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np || !(in.d[p] > (T)0)) return;
+    const uint32_t q = pos_dev[p];
+    out.d[q] = in.d[p];
+    out.x[q] = in.x[p];
+    out.y[q] = in.y[p];
+    out.z[q] = in.z[p];
+    out.u[q] = in.u[p];
+    out.v[q] = in.v[p];
+    out.w[q] = in.w[p];
+    if (in.rho) ((T *)out.rho)[q] = in.rho[p];
+    pid_out[q] = (int32_t)new_up[(uint32_t)pid_in[p]];
+}
+
+template <class T>
+hipError_t launch_compact(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out, uint32_t np,
+                          uint32_t *live_dev, uint32_t *live_up, uint32_t *pos_dev, uint32_t *new_up,
+                          uint32_t *tile_sums, int32_t *offsets_dev, int32_t *offsets_up, uint32_t *cursor,
+                          hipStream_t s) {
+    if (!np) return hipSuccess;
+    const unsigned g = (np + 255) / 256;
+    k_live_flags<T><<<g, 256, 0, s>>>(in.d, pid_in, np, live_dev, live_up);
+    hipError_t e = launch_exclusive_scan(live_dev, np, pos_dev, tile_sums, offsets_dev, cursor, s);
+    if (e == hipSuccess) e = launch_exclusive_scan(live_up, np, new_up, tile_sums, offsets_up, cursor, s);
+    if (e != hipSuccess) return e;
+    k_compact_scatter<T><<<g, 256, 0, s>>>(in, pid_in, out, pid_out, pos_dev, new_up, np);
+    return hipGetLastError();
+}
+template hipError_t launch_compact<float>(const Soa<float> &, const int32_t *, Soa<float>, int32_t *, uint32_t,
+                                          uint32_t *, uint32_t *, uint32_t *, uint32_t *, uint32_t *, int32_t *,
+                                          int32_t *, uint32_t *, hipStream_t);
+template hipError_t launch_compact<double>(const Soa<double> &, const int32_t *, Soa<double>, int32_t *, uint32_t,
+                                           uint32_t *, uint32_t *, uint32_t *, uint32_t *, uint32_t *, int32_t *,
+                                           int32_t *, uint32_t *, hipStream_t);
+
+// Count parcels whose position is outside [origin, ending) on some axis, from parcel
+// p0 on (removed parcels, d <= 0, are not checked).
+template <class T>
+__global__ void k_range_check(Soa<T> s, uint32_t p0, uint32_t np, Geo g, unsigned int *bad) {
+    const uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np || !(s.d[p] > (T)0)) return;
     const int32_t i = axis_floor((double)s.x[p], g.o[0], g.rd[0]);
     const int32_t j = axis_floor((double)s.y[p], g.o[1], g.rd[1]);
     const int32_t k = axis_floor((double)s.z[p], g.o[2], g.rd[2]);
@@ -1259,15 +1402,15 @@ __global__ void k_range_check(Soa<T> s, uint32_t np, Geo g, unsigned int *bad) {
 }
 
 template <class T>
-hipError_t launch_range_check(const Soa<T> &s, uint32_t np, Geo g, unsigned int *bad,
+hipError_t launch_range_check(const Soa<T> &s, uint32_t p0, uint32_t np, Geo g, unsigned int *bad,
                               hipStream_t st) {
-    if (!np) return hipSuccess;
-    k_range_check<T><<<(np + 255) / 256, 256, 0, st>>>(s, np, g, bad);
+    if (np <= p0) return hipSuccess;
+    k_range_check<T><<<(np - p0 + 255) / 256, 256, 0, st>>>(s, p0, np, g, bad);
     return hipGetLastError();
 }
-template hipError_t launch_range_check<float>(const Soa<float> &, uint32_t, Geo, unsigned int *,
+template hipError_t launch_range_check<float>(const Soa<float> &, uint32_t, uint32_t, Geo, unsigned int *,
                                               hipStream_t);
-template hipError_t launch_range_check<double>(const Soa<double> &, uint32_t, Geo, unsigned int *,
+template hipError_t launch_range_check<double>(const Soa<double> &, uint32_t, uint32_t, Geo, unsigned int *,
                                                hipStream_t);
 
 // Streaming copy (bandwidth calibration): each thread moves 4 independent 16-byte

@@ -4,7 +4,7 @@ Importing this package loads libscaletrack.so and fails loudly if it is not buil
 """
 from . import _lib  # noqa: F401  (loads the library)
 from .tracking import (  # noqa: F401
-    FP32, FP64, FLUSH_ATOMIC, FLUSH_OWNER, PERIODIC, PLAIN, SORTED, WALL, Chunk, Context, Graph,
+    CHANNEL, FP32, FP64, FLUSH_ATOMIC, FLUSH_OWNER, PERIODIC, PLAIN, SORTED, WALL, Chunk, Context, Graph,
     Batch, Mesh, Time, TwoWayEulerian, construct_mesh, device_count, evolve, evolve_device,
     autotune_batch, increment_time, params)
 from ._lib import ScaleTrackError  # noqa: F401

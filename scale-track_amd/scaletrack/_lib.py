@@ -21,6 +21,7 @@ ST_FP64 = 0
 ST_FP32 = 1
 ST_WALL = 0
 ST_PERIODIC = 1
+ST_CHANNEL = 2
 ST_METHOD_PLAIN = 0
 ST_METHOD_SORTED = 1
 ST_FLUSH_OWNER = 0
@@ -90,6 +91,9 @@ SIGNATURES = {
     "st_cloud_upload": (C.c_int, [_vp] + [_vp] * 8),
     "st_cloud_download": (C.c_int, [_vp] + [_vp] * 7),
     "st_cloud_count": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "st_cloud_reserve": (C.c_int, [_vp, C.c_int64]),
+    "st_cloud_inject": (C.c_int, [_vp, C.c_int64] + [_vp] * 8),
+    "st_cloud_compact": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "st_field_create": (C.c_int, [_vp, C.POINTER(st_mesh), C.c_int32, _pp]),
     "st_field_destroy": (C.c_int, [_vp]),
     "st_field_upload_U": (C.c_int, [_vp, _vp, C.c_int32]),

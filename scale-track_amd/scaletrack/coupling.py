@@ -66,11 +66,22 @@ class Events:
 
 class GpuExecutor:
     """The device side of the tracker: one st_ctx owned by the tracker thread, one
-    cloud per chunk and one field."""
+    cloud per chunk and one field.
 
-    def __init__(self, mesh, chunks_host, precision, nsub=10, params=None, device=0):
+    Extension (an inflow; the reference's chunks keep a fixed N): `injector(step)`, if
+    given, returns for parcel step `step` (1, 2, ...) one entry per chunk, None or the
+    arrays d, x, y, z, u, v, w of parcels entering before the step (st_cloud_inject);
+    with `compact_every` = k > 0 the parcels an open boundary removed are dropped after
+    every k-th step (st_cloud_compact).  `final` holds each chunk's parcels (upload
+    order) when the executor stops."""
+
+    def __init__(self, mesh, chunks_host, precision, nsub=10, params=None, device=0, injector=None,
+                 compact_every=0):
         self.mesh, self.chunks_host, self.precision = mesh, chunks_host, precision
         self.nsub, self.params, self.device = nsub, params, device
+        self.injector, self.compact_every = injector, int(compact_every)
+        self.step = 0
+        self.final = None
 
     def start(self):
         self.ctx = trk.Context(self.device)
@@ -89,9 +100,17 @@ class GpuExecutor:
         self.field.upload_U(U)
 
     def evolve(self, dt):
+        self.step += 1
+        if self.injector is not None:
+            for c, new in zip(self.chunks, self.injector(self.step)):
+                if new is not None and len(new[0]):
+                    c.inject(*new)
         boxes = []
         for c in self.chunks:
             boxes.append(trk.evolve_device(self.ctx, c, self.field, dt, self.nsub, self.params, want_bbox=True))
+        if self.compact_every > 0 and self.step % self.compact_every == 0:
+            for c in self.chunks:
+                c.compact()
         return boxes
 
     def download_source(self, out):
@@ -101,6 +120,12 @@ class GpuExecutor:
         return [c.download() for c in self.chunks]
 
     def stop(self):
+        try:
+            self.final = [[a.copy() for a in c.download()] for c in self.chunks]
+        finally:
+            self._close()
+
+    def _close(self):
         for c in self.chunks:
             c.close()
         self.field.close()
@@ -115,7 +140,7 @@ class AsyncTracker:
     the constructor, as init_async_evolve! is spawned at load time (:1625-1629)."""
 
     def __init__(self, mesh, chunks_host, precision=trk.FP32, nsub=10, params=None, device=0,
-                 executor=None, nSkipInitTimeStepsProfiling=2):
+                 executor=None, nSkipInitTimeStepsProfiling=2, injector=None, compact_every=0):
         self.mesh = mesh
         self.precision = precision
         dtype = np.float64 if precision == trk.FP64 else np.float32
@@ -127,7 +152,8 @@ class AsyncTracker:
         self.eulerian_lock = threading.Lock()
         self.chunk_transfers = threading.Lock()
         self.extrapolator = ConstExtrapolator(mesh.ncell, dtype)
-        self.executor = executor or GpuExecutor(mesh, chunks_host, precision, nsub, params, device)
+        self.executor = executor or GpuExecutor(mesh, chunks_host, precision, nsub, params, device, injector,
+                                                compact_every)
         self.iTimeStep = 0
         self.dt = None
         self.boxes = []

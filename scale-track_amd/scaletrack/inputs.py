@@ -286,6 +286,84 @@ def adjoint_case(np_parcels=1 << 20, ncell=64, nsub=200, seed=SEED, boundary=1):
                 rho_c=1000.0, dt_fluid=nsub * 1e-4, nsub=nsub, boundary=boundary, name="adjoint")
 
 
+def channel_profile(n, origin, ending, u0=1.0):
+    """Laminar duct inflow with walls on y and z: u = (9/4) u0 16 eta (1 - eta) zeta (1 - zeta)
+    at cell centres (eta, zeta the y, z coordinates scaled to [0, 1]), whose mean over the
+    cross-section is u0; v = w = 0.  AoS (ncell, 3)."""
+    x, y, z = cell_centres(n, origin, ending)
+    eta = (y - origin[1]) / (ending[1] - origin[1])
+    zeta = (z - origin[2]) / (ending[2] - origin[2])
+    U = np.zeros((x.size, 3))
+    U[:, 0] = 2.25 * u0 * 16.0 * eta * (1.0 - eta) * zeta * (1.0 - zeta)
+    return U
+
+
+def channel_field(n, origin, ending, u0=1.0, perturbation=0.05, seed=SEED, dtype=np.float64):
+    """The channel's frozen carrier: the laminar profile plus a divergence-free random
+    perturbation (8 Fourier modes) of RMS perturbation * u0."""
+    U = channel_profile(n, origin, ending, u0)
+    P = fourier_field(n, origin, ending, modes=8, seed=seed, stream=70)
+    rms = float(np.sqrt((P ** 2).sum(1).mean()))
+    if rms > 0:
+        U += P * (perturbation * u0 / rms)
+    return U.astype(dtype)
+
+
+def cells_of(case, x, y, z):
+    """Linear cell index (x fastest) of positions inside the mesh."""
+    idx = []
+    for a, c in enumerate((x, y, z)):
+        d = (case.ending[a] - case.origin[a]) / case.n[a]
+        idx.append(np.clip(np.floor((c.astype(np.float64) - case.origin[a]) * (1.0 / d)), 0, case.n[a] - 1)
+                   .astype(np.int64))
+    return idx[2] * case.n[0] * case.n[1] + idx[1] * case.n[0] + idx[0]
+
+
+def channel_case(np_parcels=1 << 25, n=(256, 128, 128), length=(2.0, 1.0, 1.0), u0=1.0, perturbation=0.05,
+                 dtype=np.float32, seed=SEED, dt_fluid=1e-3, nsub=10, name="channel"):
+    """The channel workload at one GPU's shape: n cells (256 x 128 x 128, uniform cubes of
+    1/128 m) on a 2 x 1 x 1 m duct, walls on y and z, open inlet (x = 0) and outlet
+    (x = 2 m): boundary form 2 (ST_CHANNEL).  Frozen carrier: laminar profile of bulk
+    velocity u0 with seeded 5% perturbations.  np_parcels parcels fill the duct at
+    seeded uniform positions -- the state the seeded inflow builds up -- moving with
+    the carrier velocity of their cell, log-normal diameters (median 30 um, sigma 0.5),
+    rho_p = 1000 kg/m^3; carrier mu = 1e-3, rho = 1e3."""
+    o = (0.0, 0.0, 0.0)
+    e = tuple(float(v) for v in length)
+    U = channel_field(n, o, e, u0, perturbation, seed, dtype=dtype)
+    x, y, z = positions(seed, np_parcels, o, e, dtype=dtype)
+    d = lognormal_diameters(seed, np_parcels, dtype=dtype)
+    c = Case(n, o, e, U, d, x, y, z, np.zeros(np_parcels, dtype), np.zeros(np_parcels, dtype),
+             np.zeros(np_parcels, dtype), mu_c=1e-3, rho_d=1000.0, rho_c=1000.0, dt_fluid=dt_fluid, nsub=nsub,
+             boundary=2, name=name)
+    cell = cells_of(c, c.x, c.y, c.z)
+    c.u, c.v, c.w = [np.ascontiguousarray(U[cell, k]) for k in range(3)]
+    return c
+
+
+def channel_injection(case, step, count, seed=SEED):
+    """The parcels entering the channel during fluid step `step`: count parcels spread
+    uniformly over the inlet plane, each placed at x = origin + xi u dt (the distance the
+    carrier at its inlet cell moves it during the step, xi uniform in [0, 1)) so the
+    inflow rate is uniform in time, moving with that carrier velocity, with log-normal
+    diameters; seeded by (seed, step).  Returns d, x, y, z, u, v, w in the case's dtype."""
+    dt = case.d.dtype
+    s = seed * 1000003 + step
+    yz = [case.origin[a] + uniform(s, 80 + a, count) * (case.ending[a] - case.origin[a]) for a in (1, 2)]
+    d1 = (case.ending[0] - case.origin[0]) / case.n[0]
+    x0 = np.full(count, case.origin[0] + 0.5 * d1)
+    cell = cells_of(case, x0, yz[0], yz[1])
+    uin = case.U[cell].astype(np.float64)
+    xi = uniform(s, 83, count)
+    x = case.origin[0] + xi * np.maximum(uin[:, 0], 0.0) * case.dt_fluid
+    x = np.minimum(x, case.origin[0] + 0.999 * d1)
+    y = into_cells(yz[0].astype(dt), case.origin[1], case.ending[1], case.n[1])
+    z = into_cells(yz[1].astype(dt), case.origin[2], case.ending[2], case.n[2])
+    x = into_cells(x.astype(dt), case.origin[0], case.ending[0], case.n[0])
+    d = lognormal_diameters(s, count, dtype=dt, stream=84)
+    return [d, x, y, z] + [np.ascontiguousarray(uin[:, k].astype(dt)) for k in range(3)]
+
+
 def target_source(case, seed=SEED, scale=1e-12):
     """Seeded target field of the source-misfit loss L = |S - T|^2."""
     return scale * normal(seed, 60, case.ncell * 3).reshape(case.ncell, 3)

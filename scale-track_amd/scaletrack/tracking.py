@@ -17,7 +17,7 @@ from . import _lib
 from ._lib import check, lib
 
 FP64, FP32 = _lib.ST_FP64, _lib.ST_FP32
-WALL, PERIODIC = _lib.ST_WALL, _lib.ST_PERIODIC
+WALL, PERIODIC, CHANNEL = _lib.ST_WALL, _lib.ST_PERIODIC, _lib.ST_CHANNEL
 PLAIN, SORTED = _lib.ST_METHOD_PLAIN, _lib.ST_METHOD_SORTED
 FLUSH_OWNER, FLUSH_ATOMIC = _lib.ST_FLUSH_OWNER, _lib.ST_FLUSH_ATOMIC
 
@@ -211,9 +211,52 @@ class Chunk:
         check(lib.st_cloud_upload(self.handle, *[_ptr(a) for a in arrs], _ptr(self.rho_p)))
 
     def download(self):
+        """Device parcels into the host arrays, in upload order (parcels an open boundary
+        removed come back with d < 0)."""
+        n = self.count()
+        if n != self.N:
+            self._resize(n)
         arrs = [getattr(self, f) for f in self.FIELDS]
         check(lib.st_cloud_download(self.handle, *[_ptr(a) for a in arrs]))
         return arrs
+
+    def count(self):
+        n = C.c_int64()
+        check(lib.st_cloud_count(self.handle, C.byref(n)))
+        return n.value
+
+    def _resize(self, n):
+        dt = _dtype(self.precision)
+        for f in self.FIELDS:
+            setattr(self, f, np.zeros(n, dtype=dt))
+        self.N = n
+
+    def reserve(self, capacity):
+        check(lib.st_cloud_reserve(self.handle, int(capacity)))
+
+    def inject(self, d, x, y, z, u, v, w, rho=None):
+        """Append parcels after the current ones (st_cloud_inject); they take the next
+        upload-order indices."""
+        dt = _dtype(self.precision)
+        arrs = [np.ascontiguousarray(a, dtype=dt) for a in (d, x, y, z, u, v, w)]
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=dt)
+        k = arrs[0].size
+        check(lib.st_cloud_inject(self.handle, k, *[_ptr(a) for a in arrs], _ptr(r)))
+        for f, a in zip(self.FIELDS, arrs):
+            setattr(self, f, np.concatenate([getattr(self, f), a]))
+        if r is not None and self.rho_p is not None:
+            self.rho_p = np.concatenate([self.rho_p, r])
+        self.N += k
+
+    def compact(self):
+        """Drop the parcels an open boundary removed (st_cloud_compact); returns how many
+        are left.  The host arrays are resized (their contents are refreshed by the next
+        download)."""
+        n = C.c_int64()
+        check(lib.st_cloud_compact(self.handle, C.byref(n)))
+        if n.value != self.N:
+            self._resize(n.value)
+        return n.value
 
     def sort(self, ctx, eulerian, p=None):
         """Sort the device parcels by brick of `eulerian`'s mesh now (st_cloud_sort);
