@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-parcels", type=int, default=1 << 22)
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (JSON lines)")
-    ap.add_argument("--workload", choices=["forward", "batch", "adjoint"], default="forward",
+    ap.add_argument("--workload", choices=["forward", "batch", "adjoint", "channel"], default="forward",
                     help="with --only: which workload to time (profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the streaming copy kernel over BYTES (traffic-counter calibration)")
@@ -275,6 +275,53 @@ def batch_workload(stk, inputs, ctx, nclouds=64, steps=5, warmup=1, seed=19891, 
             "tuned": best, "tune_s": t_tune, "dtype": "f32 state, f64 face test and source"}
 
 
+def channel_workload(stk, inputs, ctx, steps=5, warmup=1, np_parcels=1 << 25, inject_per_step=(1 << 25) // 100):
+    """The channel at one GPU's shape: 2^25 parcels filling a 2 x 1 x 1 m duct of
+    256 x 128 x 128 cells (walls on y and z, open inlet and outlet, frozen laminar
+    carrier with 5% perturbations), FP32 state with FP64 face test and source.  One
+    fluid step = the inflow's parcels injected at the inlet (2^25 / 100 per step, host
+    arrays generated beforehand), UTrans zeroed, 10 one-substep passes."""
+    case = inputs.channel_case(np_parcels=np_parcels, dtype=np.float32)
+    field = stk.TwoWayEulerian(ctx, stk.Mesh(case.n, case.origin, case.ending), stk.FP32)
+    field.upload_U(case.U)
+    chunk = stk.Chunk(ctx, case.np, case.mu_c, case.rho_d, case.rho_d_by_rho_c, stk.FP32)
+    chunk.set_state(*case.state())
+    chunk.upload()
+    chunk.reserve(case.np + (steps + warmup) * inject_per_step)
+    news = [inputs.channel_injection(case, k + 1, inject_per_step) for k in range(steps + warmup)]
+    p = stk.params(boundary=stk.CHANNEL, substeps_per_pass=1)
+    for k in range(warmup):
+        chunk.inject(*news[k])
+        field.zero_source()
+        stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
+    resorts0 = chunk.resort_count()
+    n0 = chunk.count()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        chunk.inject(*news[k])
+        field.zero_source()
+        stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    timed = ctx.kernel_times(steps)
+    kms = float(np.mean([ms for ms, _ in timed]))
+    n1 = chunk.count()
+    removed = int((chunk.download()[0] < 0).sum())
+    resorts = chunk.resort_count() - resorts0
+    chunk.close()
+    field.close()
+    updates = sum(n0 + (k + 1) * inject_per_step for k in range(steps)) * case.nsub
+    bpp = bytes_per_pass((n0 + n1) // 2, case.ncell, True)
+    return {"parcels_start": n0, "parcels_end": n1, "removed_total": removed, "inject_per_step": inject_per_step,
+            "cells": list(case.n), "substeps_per_step": case.nsub, "updates_per_s": updates / el,
+            "ms_per_step": 1e3 * el / steps, "kernel_ms_per_pass": kms,
+            "roofline_frac": bpp / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS, "resorts_in_timed_steps": resorts,
+            "dtype": "f32 state, f64 face test and source",
+            "timing": "wall clock of whole fluid steps: injection (host arrays over PCIe), zeroing, passes and "
+                      "the sorts the injected parcels trigger; kernel_ms_per_pass from HIP events around the passes"}
+
+
 def adjoint_workload(stk, inputs, ctx, np_parcels=1 << 20, ncell=64, nsub=200, K=20, seed=19891):
     """Forward + adjoint: frozen divergence-free field of 8 seeded Fourier modes, per-
     parcel d and rho, nsub substeps of 1e-4 s with a checkpoint every K; loss = L2 misfit
@@ -383,6 +430,9 @@ def main():
     if args.only and args.workload == "adjoint":
         print(json.dumps(adjoint_workload(stk, inputs, ctx)))
         return
+    if args.only and args.workload == "channel":
+        print(json.dumps(channel_workload(stk, inputs, ctx, steps=args.steps, warmup=args.warmup)))
+        return
     t0 = time.perf_counter()
     case = inputs.taylor_green_box(np_parcels=args.parcels, ncell=args.cells, boundary=stk.PERIODIC)
     print(f"# inputs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -472,6 +522,8 @@ def main():
     if not args.only:
         line["batch"] = batch_workload(stk, inputs, ctx, tune_cache=args.tune_cache)
         print(f"# batch: {json.dumps(line['batch'])}", file=sys.stderr, flush=True)
+        line["channel"] = channel_workload(stk, inputs, ctx)
+        print(f"# channel: {json.dumps(line['channel'])}", file=sys.stderr, flush=True)
         line["adjoint"] = adjoint_workload(stk, inputs, ctx)
         line["adjoint_16M"] = adjoint_workload(stk, inputs, ctx, np_parcels=1 << 24, ncell=128, nsub=50, K=10)
         print(f"# adjoint: {json.dumps(line['adjoint'])}", file=sys.stderr, flush=True)

@@ -127,7 +127,8 @@ int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
     st_params p;
     ST_TRY(prepare(a->ctx, c, f, dt_fluid, a->nsub, &a->p, p));
     hipStream_t s = a->ctx->stream;
-    if (p.method == ST_METHOD_SORTED && !sort_matches(c, f, p)) ST_TRY(sort_any(c, f, p));
+    // sorted, and with no unsorted tail: the checkpoints follow the sorted passes
+    if (p.method == ST_METHOD_SORTED && (!sort_matches(c, f, p) || c->sorted_n != c->n)) ST_TRY(sort_any(c, f, p));
     ST_TRY(st_field_zero_source(f));
     StatsBuf sb;
     ST_TRY(cloud_stats(c, stats_slots(c, p), sb));

@@ -222,6 +222,39 @@ static bool same_geometry(const st_cloud *c, const st_field *f, const st_params 
     return true;
 }
 
+// The sort buffers for nbr bricks and the cloud's capacity: grown (when may_alloc) or
+// only checked (ST_E_STATE when too small).
+static int sort_buffers(st_cloud *c, int64_t nbr, bool may_alloc) {
+    const int32_t rows = sort_rows(c->n, nbr);
+    const size_t n1 = (size_t)(c->cap > 0 ? c->cap : 1);
+    if (!may_alloc) {
+        const bool ok = c->counts && c->cursor && c->brick_start && c->bins_cap >= nbr && c->keys &&
+                        c->keys_cap >= n1 && (rows == 0 || (c->hist && c->hist_cap >= (size_t)rows * (size_t)nbr)) &&
+                        c->buf[1 - c->cur][0] && c->pid[1 - c->cur];
+        return ok ? ST_OK
+                  : fail(ST_E_STATE, "the cloud's sort buffers do not hold its %lld parcels; a sort (st_cloud_sort or "
+                                     "st_evolve) sizes them",
+                         (long long)c->n);
+    }
+    size_t bins = (size_t)c->bins_cap, bins1 = (size_t)c->bins_cap + 1;
+    size_t bins_c = bins;
+    ST_TRY(grow(c, &c->counts, bins, (size_t)nbr, "sort counters"));
+    ST_TRY(grow(c, &c->cursor, bins_c, (size_t)nbr, "sort cursors"));
+    ST_TRY(grow(c, &c->brick_start, bins1, (size_t)nbr + 1, "brick ranges"));
+    c->bins_cap = (int32_t)std::max<int64_t>(c->bins_cap, nbr);
+    ST_TRY(grow(c, &c->keys, c->keys_cap, n1, "sort keys"));
+    if (rows > 0) ST_TRY(grow(c, &c->hist, c->hist_cap, (size_t)rows * (size_t)nbr, "sort histograms"));
+    return alloc_set(c, 1 - c->cur);
+}
+
+bool sort_buffers_fit(const st_cloud *c) {
+    const int32_t rows = sort_rows(c->n, c->nbricks);
+    const size_t n1 = (size_t)(c->cap > 0 ? c->cap : 1);
+    return c->counts && c->cursor && c->brick_start && c->bins_cap >= c->nbricks && c->keys && c->keys_cap >= n1 &&
+           (rows == 0 || (c->hist && c->hist_cap >= (size_t)rows * (size_t)c->nbricks)) && c->buf[1 - c->cur][0] &&
+           c->pid[1 - c->cur];
+}
+
 template <class T>
 int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
     int32_t nb[3];
@@ -237,19 +270,11 @@ int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
                     "mesh; a sort by another brick or mesh would change what they step: destroy them first",
                     c->pins, c->sort_bd[0], c->sort_bd[1], c->sort_bd[2]);
     const int32_t rows = sort_rows(c->n, nbr);
-    const size_t n1 = (size_t)(c->cap > 0 ? c->cap : 1);
-    size_t bins = (size_t)c->bins_cap, bins1 = (size_t)c->bins_cap + 1;
-    size_t bins_c = bins;
-    ST_TRY(grow(c, &c->counts, bins, (size_t)nbr, "sort counters"));
-    ST_TRY(grow(c, &c->cursor, bins_c, (size_t)nbr, "sort cursors"));
-    ST_TRY(grow(c, &c->brick_start, bins1, (size_t)nbr + 1, "brick ranges"));
-    c->bins_cap = (int32_t)std::max<int64_t>(c->bins_cap, nbr);
-    ST_TRY(grow(c, &c->keys, c->keys_cap, n1, "sort keys"));
-    if (rows > 0) ST_TRY(grow(c, &c->hist, c->hist_cap, (size_t)rows * (size_t)nbr, "sort histograms"));
-    ST_TRY(alloc_set(c, 1 - c->cur));
+    ST_TRY(sort_buffers(c, nbr, true));
     ST_TRY(enqueue_sort<T>(c, f, p, nb, nbr, rows, nullptr, nullptr));
     c->permuted = true;
     c->sorted = true;
+    c->sorted_n = c->n;
     for (int a = 0; a < 3; ++a) {
         c->sort_n[a] = f->mesh.n[a];
         c->sort_bd[a] = p.brick[a];
@@ -320,8 +345,24 @@ static int multi_grid(const st_ctx *ctx, const st_cloud *c, const st_params &p, 
     return (int)std::min<int64_t>((int64_t)per_cu * ctx->cus, std::max<int32_t>(1, c->nbricks));
 }
 
+int32_t tail_blocks(const st_cloud *c, const st_params &p) {
+    if (p.method != ST_METHOD_SORTED || c->n <= c->sorted_n) return 0;
+    return (int32_t)((c->n - c->sorted_n + p.block - 1) / p.block);
+}
+
 int32_t stats_slots(const st_cloud *c, const st_params &p) {
-    return p.method == ST_METHOD_SORTED ? c->nbricks : (int32_t)((c->n + p.block - 1) / p.block);
+    return p.method == ST_METHOD_SORTED ? c->nbricks + tail_blocks(c, p) : (int32_t)((c->n + p.block - 1) / p.block);
+}
+
+// Takes in the last step's statistics once they have arrived, without waiting: the
+// stray share, and -- when that step's re-sort ran -- that every parcel up to the count
+// it sorted is in brick order again (sorted_n only ever grows to a count a re-sort is
+// known to have covered, so the tail bound never falls short).
+static void observe_step(st_cloud *c) {
+    if (!c->strays_pending || hipEventQuery(c->strays_ev) != hipSuccess) return;
+    c->strays_pending = false;
+    c->last_stray_fraction = c->strays_den > 0 ? (double)c->strays_host[0] / (double)c->strays_den : 0.0;
+    if (c->strays_host[1] && c->sorted) c->sorted_n = std::max(c->sorted_n, c->strays_den);
 }
 
 int cloud_stats(st_cloud *c, int32_t nslots, StatsBuf &sb) {
@@ -338,6 +379,7 @@ template <class T>
 int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p,
                bool capturing, const StatsBuf &sb, bool device_resort, T *ck) {
     hipStream_t s = ctx->stream;
+    if (!capturing) observe_step(c);
     PassArgs<T> a{};
     a.s = soa<T>(c, c->cur);
     a.U = (const T *)f->U;
@@ -357,8 +399,12 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     a.np = (uint32_t)c->n;
     const int per = p.substeps_per_pass > 0 ? p.substeps_per_pass : nsub;
     const int bnd = p.boundary;
-    // passes of one persistent launch (sorted method, no checkpoint)
-    const int group = p.method == ST_METHOD_SORTED && !ck ? std::max(1, p.passes_per_launch) : 1;
+    // the unsorted tail of a sorted cloud (parcels injected since its last sort)
+    const int32_t tblocks = tail_blocks(c, p);
+    PassArgs<T> ta = a;
+    ta.tail0 = c->brick_start + c->nbricks;
+    // passes of one persistent launch (sorted method, no checkpoint, no tail)
+    const int group = p.method == ST_METHOD_SORTED && !ck && tblocks == 0 ? std::max(1, p.passes_per_launch) : 1;
     const int grid = group > 1 ? multi_grid<T>(ctx, c, p, a.tile_cells) : 0;
     if (group > 1 && grid < 1) return fail(ST_E_RANGE, "the multi-pass kernel does not fit: block %d", p.block);
 
@@ -384,12 +430,24 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
             ST_HIP(launch_pass_sorted<T>(a, bnd, p.block, s));
         else
             ST_HIP(launch_pass_plain<T>(a, bnd, p.block, s));
+        if (tblocks > 0) {  // after the sorted launch: its owner flush is done
+            ta.nsteps = a.nsteps;
+            ta.part = a.part ? a.part + (size_t)c->nbricks * kStatSlot : nullptr;
+            ST_HIP(launch_pass_plain<T>(ta, bnd, p.block, s, (uint32_t)tblocks));
+        }
         passes += k;
         done = next;
     }
     // the recorded time covers every pass; per-pass figures divide by the pass count
     if (!capturing) ST_HIP(timer_stop(ctx, s, passes));
-    const bool resort = device_resort && p.method == ST_METHOD_SORTED && c->sorted && c->n > 0;
+    bool resort = device_resort && p.method == ST_METHOD_SORTED && c->sorted && c->n > 0;
+    if (resort) {
+        const int r = sort_buffers(c, c->nbricks, sb.may_alloc);
+        if (r != ST_OK) {
+            if (sb.may_alloc) return r;
+            resort = false;  // no room to re-sort without allocating: the tail stays
+        }
+    }
     StatsTarget t{sb.part, nslots, sb.bbox, sb.strays, nullptr, 0u};
     if (resort) {
         // share > resort_fraction  <=>  strays > floor(resort_fraction * n)
@@ -399,13 +457,16 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     }
     ST_HIP(launch_reduce_stats(t, s));
     if (p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER) f->stray_dirty = true;
+    if (resort) ST_TRY(queue_resort<T>(c, f, p, sb.gate));
     if (p.method == ST_METHOD_SORTED && !capturing) {
         ST_HIP(hipMemcpyAsync(c->strays_host, sb.strays, 4, hipMemcpyDeviceToHost, s));
+        // the gate this step's re-sort ran on, or a device word that is always 0 (the
+        // host never writes the pinned words: an earlier copy may still be in flight)
+        ST_HIP(hipMemcpyAsync(c->strays_host + 1, resort ? sb.gate : c->resort + 2, 4, hipMemcpyDeviceToHost, s));
         ST_HIP(hipEventRecord(c->strays_ev, s));
         c->strays_pending = true;
         c->strays_den = c->n;
     }
-    if (resort) ST_TRY(queue_resort<T>(c, f, p, sb.gate));
     ++c->epoch;
     return ST_OK;
 }
@@ -618,7 +679,7 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->bad, 4);
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->resort, 4 * sizeof(unsigned int));
     if (r == ST_OK && e == hipSuccess) e = st::launch_fill_u32(c->resort, 0u, 4, ctx->stream);
-    if (r == ST_OK && e == hipSuccess) e = hipHostMalloc((void **)&c->strays_host, 4, hipHostMallocDefault);
+    if (r == ST_OK && e == hipSuccess) e = hipHostMalloc((void **)&c->strays_host, 8, hipHostMallocDefault);
     if (r == ST_OK && e == hipSuccess) e = hipEventCreateWithFlags(&c->strays_ev, hipEventDisableTiming);
     if (r == ST_OK && e == hipSuccess) e = st::launch_iota(c->pid[0], (uint32_t)n, ctx->stream);
     if (r != ST_OK || e != hipSuccess) {
@@ -802,7 +863,8 @@ int st_cloud_inject(st_cloud *c, int64_t k, const void *d, const void *x, const 
     c->fresh = true;
     c->n = n1;
     c->uploaded = true;
-    c->sorted = false;  // the new parcels sit at the end; the next sorted step sorts
+    // the new parcels sit at the end: sorted steps step them as the unsorted tail until
+    // a re-sort takes them in
     ++c->epoch;
     return ST_OK;
 }
@@ -1011,8 +1073,7 @@ int st_cloud_stray_fraction(st_cloud *c, double *fraction) {
     ST_TRY(check_ctx(c->ctx));
     if (c->strays_pending) {
         ST_HIP(hipEventSynchronize(c->strays_ev));
-        c->strays_pending = false;
-        c->last_stray_fraction = c->strays_den > 0 ? (double)*c->strays_host / (double)c->strays_den : 0.0;
+        observe_step(c);
     }
     *fraction = c->last_stray_fraction;
     return ST_OK;
@@ -1046,6 +1107,7 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
     if (p.method == ST_METHOD_SORTED && !sort_matches(c, f, p))
         ST_TRY(c->prec == ST_FP64 ? sort_cloud<double>(c, f, p) : sort_cloud<float>(c, f, p));
     // every allocation happens before the capture: none is permitted inside it
+    if (p.method == ST_METHOD_SORTED) ST_TRY(sort_buffers(c, c->nbricks, true));
     StatsBuf sb;
     ST_TRY(cloud_stats(c, stats_slots(c, p), sb));
     ST_HIP(hipStreamSynchronize(ctx->stream));

@@ -88,13 +88,19 @@ struct st_cloud {
     int32_t part_cap = 0;
     unsigned int *strays = nullptr;      // device
     unsigned int *bad = nullptr;         // device
-    unsigned int *strays_host = nullptr; // pinned
+    unsigned int *strays_host = nullptr; // pinned: [0] strays of the last step, [1] its re-sort gate
     hipEvent_t strays_ev = nullptr;
     bool strays_pending = false;
-    int64_t strays_den = 0;  // parcels x passes behind strays_host
+    int64_t strays_den = 0;  // parcels behind strays_host
+    // Parcels [0, sorted_n) were in brick order after the last sort the host knows of;
+    // parcels injected since then form an unsorted tail [brick_start[nbricks], n) that
+    // each pass steps with a plain launch.  The device re-sorts fold the tail in; the
+    // host learns of one when it reads the gate back, so sorted_n may lag (the tail
+    // launch reads its start from the device and covers n - sorted_n parcels at most).
+    int64_t sorted_n = 0;
     double last_stray_fraction = -1.0;
     // device: [0] gate of the re-sort queued behind the last step (set by the statistics
-    // reduction), [1] re-sorts the device decided on so far
+    // reduction), [1] re-sorts the device decided on so far, [2] always 0
     unsigned int *resort = nullptr;
     // Graphs, batches and adjoints holding raw device pointers of this cloud: while any
     // is alive the cloud refuses to be destroyed, re-sorted with another brick geometry,
@@ -158,11 +164,17 @@ struct StatsBuf {
     double *bbox = nullptr;
     unsigned int *strays = nullptr;
     unsigned int *gate = nullptr;
+    bool may_alloc = true;  // the device re-sort may size the cloud's sort buffers first
 };
 // The cloud's own statistics buffers (slots grown to nslots on demand).
 int cloud_stats(st_cloud *c, int32_t nslots, StatsBuf &sb);
-// Statistics slots one step of the cloud uses with these params.
+// Statistics slots one step of the cloud uses with these params (sorted: one per brick,
+// then one per workgroup of the unsorted tail; plain: one per workgroup).
 int32_t stats_slots(const st_cloud *c, const st_params &p);
+// Workgroups of the tail launch of a sorted step (0: no tail).
+int32_t tail_blocks(const st_cloud *c, const st_params &p);
+// Whether the sort buffers hold the cloud's current parcel count and brick geometry.
+bool sort_buffers_fit(const st_cloud *c);
 // Queues nsub substeps of length dt_sub (exact in T) on the context stream, split in
 // passes of p.substeps_per_pass.  With ck (sorted method, FP64) the first pass also
 // writes the state each parcel starts with into ck (6 arrays of c->n, device order).

@@ -80,6 +80,9 @@ struct PassArgs {
     int32_t nsteps;
     int32_t flush;   // 0 owner, 1 atomic
     uint32_t np;
+    // plain launches over a sorted cloud's unsorted tail: the first parcel is read from
+    // here (the brick ranges' end, brick_start[nbricks]); nullptr otherwise
+    const int32_t *tail0;
 };
 
 // Order-preserving map of a double onto uint64 (for atomicMin/atomicMax).
@@ -108,7 +111,7 @@ namespace st {
 template <class T>
 hipError_t launch_pass_sorted(const PassArgs<T> &a, int bnd, int block, hipStream_t s);
 template <class T>
-hipError_t launch_pass_plain(const PassArgs<T> &a, int bnd, int block, hipStream_t s);
+hipError_t launch_pass_plain(const PassArgs<T> &a, int bnd, int block, hipStream_t s, uint32_t grid = 0);
 // npass passes of a.nsteps substeps each in one persistent launch of `grid` workgroups
 // (each owning the same bricks in every pass); a.part is filled in the last pass.
 template <class T>

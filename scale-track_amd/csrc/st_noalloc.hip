@@ -91,6 +91,7 @@ extern "C" int st_evolve_noalloc(st_ctx *ctx, st_cloud *c, st_field *f, double d
     sb.gate = sb.strays + 1;
     sb.part = (double *)(base + kHeaderBytes);
     sb.cap = kMaxSlots;
+    sb.may_alloc = false;  // a re-sort runs only if the cloud's sort buffers already fit
     const double dt_sub = substep_dt(c->prec, dt_fluid, nsub);
     if (c->prec == ST_FP64)
         ST_TRY(run_passes<double>(ctx, c, f, dt_sub, nsub, p, false, sb, true));

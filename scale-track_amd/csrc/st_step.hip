@@ -160,6 +160,14 @@ struct Box {
     }
 };
 
+// Marks a parcel removed (channel form): its diameter stored negated.  The diameter is
+// read again here rather than kept in a register through the substeps, on this rare
+// path only (the wait for it also waits for the loads in flight).
+template <class T>
+__device__ __forceinline__ void mark_removed(const Soa<T> &s, uint32_t off) {
+    st_off(s.d, off, -ld_off(s.d, off));
+}
+
 struct NoPrefetch {
     __device__ __forceinline__ void operator()() const {}
 };
@@ -240,7 +248,7 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     st_off(s.w, off, vz);
     env.deposit(cell, sl, m * (s0x - (double)vx), m * (s0y - (double)vy), m * (s0z - (double)vz));
     if (BND == kChannel && exited)
-        st_off(s.d, off, -in.d);
+        mark_removed(s, off);
     else
         box.add(px, py, pz);  // :1362
 }
@@ -326,7 +334,7 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
     st_off(s.w, off, nvz);
     env.deposit(cell, sl, m * (s0x - (double)nvx), m * (s0y - (double)nvy), m * (s0z - (double)nvz));
     if (BND == kChannel && exited)
-        st_off(s.d, off, -in.d);
+        mark_removed(s, off);
     else if (stats)
         box.add(nx, ny, nz);  // :1362
 }
@@ -366,21 +374,26 @@ __device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, double
     }
 }
 
+// One thread per parcel, from parcel 0 or, over a sorted cloud's unsorted tail, from
+// tail0[0] on (each live tail parcel counts as outside its brick in the statistics).
 template <class T, int BND, bool RHO>
 __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t base = a.tail0 ? (uint32_t)a.tail0[0] : 0u;
+    const uint32_t p = base + blockIdx.x * blockDim.x + threadIdx.x;
     Box<T> box;
     box.init();
     GlobalEnv<T> env{a.U, a.src};
+    uint32_t tail = 0;
     if (p < a.np) {
         ParcelIn<T> in;
         load_parcel<T, RHO>(a.s, p, a.ph.rho, in);
+        if (a.tail0 && in.d > (T)0) tail = 1;
         if (a.nsteps == 1)
             step_parcel_one<T, BND, RHO>(a.g, a.ph, a.s, in, p, env, box, true, NoPrefetch{});
         else
             step_parcel<T, BND, RHO>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
     }
-    if (a.part) flush_stats(box, 0u, a.part + (size_t)blockIdx.x * kStatSlot);
+    if (a.part) flush_stats(box, tail, a.part + (size_t)blockIdx.x * kStatSlot);
 }
 
 // The adjoint's checkpoint of a parcel (the state it starts a segment with), written
@@ -758,9 +771,9 @@ template int batch_max_blocks_per_cu<float>(int, int, int, size_t);
 template int batch_max_blocks_per_cu<double>(int, int, int, size_t);
 
 template <class T>
-hipError_t launch_pass_plain(const PassArgs<T> &a, int bnd, int block, hipStream_t s) {
+hipError_t launch_pass_plain(const PassArgs<T> &a, int bnd, int block, hipStream_t s, uint32_t grid) {
     if (a.np == 0) return hipSuccess;
-    const uint32_t grid = (a.np + block - 1) / block;
+    if (grid == 0) grid = (a.np + block - 1) / block;
     with_forms(bnd, a.s.rho != nullptr, [&](auto B, auto R) {
         k_pass_plain<T, decltype(B)::value, decltype(R)::value><<<grid, block, 0, s>>>(a);
     });
@@ -784,8 +797,8 @@ hipError_t launch_pass_sorted(const PassArgs<T> &a, int bnd, int block, hipStrea
     return hipGetLastError();
 }
 
-template hipError_t launch_pass_plain<float>(const PassArgs<float> &, int, int, hipStream_t);
-template hipError_t launch_pass_plain<double>(const PassArgs<double> &, int, int, hipStream_t);
+template hipError_t launch_pass_plain<float>(const PassArgs<float> &, int, int, hipStream_t, uint32_t);
+template hipError_t launch_pass_plain<double>(const PassArgs<double> &, int, int, hipStream_t, uint32_t);
 template hipError_t launch_pass_sorted<float>(const PassArgs<float> &, int, int, hipStream_t);
 template hipError_t launch_pass_sorted<double>(const PassArgs<double> &, int, int, hipStream_t);
 

@@ -242,11 +242,9 @@ class Chunk:
         r = None if rho is None else np.ascontiguousarray(rho, dtype=dt)
         k = arrs[0].size
         check(lib.st_cloud_inject(self.handle, k, *[_ptr(a) for a in arrs], _ptr(r)))
-        for f, a in zip(self.FIELDS, arrs):
-            setattr(self, f, np.concatenate([getattr(self, f), a]))
+        # the host arrays are resized (not refilled) by the next download
         if r is not None and self.rho_p is not None:
             self.rho_p = np.concatenate([self.rho_p, r])
-        self.N += k
 
     def compact(self):
         """Drop the parcels an open boundary removed (st_cloud_compact); returns how many

@@ -114,6 +114,52 @@ def test_inject_and_compact(ctx, precision):
     field.close()
 
 
+@pytest.mark.parametrize("precision", [stk.FP64, stk.FP32])
+@pytest.mark.parametrize("resort_fraction", [0.02, 1.0])
+def test_injected_tail(ctx, precision, resort_fraction):
+    """Parcels injected into a sorted cloud are stepped as its unsorted tail (a plain
+    launch behind each sorted pass) until a re-sort decided on the device takes them in
+    (fraction 0.02: within a few steps; 1.0: never, the tail grows for the whole run).
+    Steps are queued back to back; every step's parcels and the summed source equal the
+    oracle's.  A graph captured with a tail and the no-allocation step take part too."""
+    dt = np.float64 if precision == stk.FP64 else np.float32
+    case = inputs.channel_case(np_parcels=20000, n=(32, 16, 16), length=(1.0, 1.0, 1.0), dtype=dt, dt_fluid=1e-2)
+    chunk, field = device(ctx, case, precision)
+    p = stk.params(boundary=stk.CHANNEL, substeps_per_pass=1, brick=(8, 4, 4), resort_fraction=resort_fraction)
+    st = case.state()
+    src = np.zeros((case.ncell, 3))
+    from scaletrack import _lib
+    import ctypes as C
+    for k in range(1, 9):
+        new = inputs.channel_injection(case, k, 1500)
+        chunk.inject(*new)
+        st = [np.concatenate([a, b]) for a, b in zip(st, new)]
+        for _ in range(2):
+            if k == 4:  # the no-allocation step on a cloud with a tail
+                _lib.check(_lib.lib.st_evolve_noalloc(ctx.handle, chunk.handle, field.handle, case.dt_fluid,
+                                                      case.nsub, C.byref(p), None))
+            else:
+                stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
+            s, _ = oracle_step(case, st, precision)
+            src += s
+        if k == 6:  # a captured step with the tail, replayed twice
+            g = stk.Graph(ctx, chunk, field, case.dt_fluid, case.nsub, p, zero_source=False)
+            for _ in range(2):
+                g.launch()
+                s, _ = oracle_step(case, st, precision)
+                src += s
+            g.close()
+    got = chunk.download()
+    assert_step(got, field.download_source(), None, st, src, None, f"fraction {resort_fraction}")
+    resorts = chunk.resort_count()
+    if resort_fraction < 1:
+        assert resorts >= 2, resorts
+    else:
+        assert resorts == 0
+    chunk.close()
+    field.close()
+
+
 def test_async_tracker_channel_100_steps(ctx):
     """The async tracker drives the channel for 100 fluid steps with an inflow of 400
     parcels per step and compaction every 5 steps: after every evolve_cloud the solver
