@@ -72,7 +72,7 @@ class Adjoint:
 
     def close(self):
         if self.handle:
-            lib.st_adjoint_destroy(self.handle)
+            check(lib.st_adjoint_destroy(self.handle))
             self.handle = None
 
     def __del__(self):
@@ -86,15 +86,26 @@ def source_misfit_rrule(ctx, chunk, field, target, dt_fluid, nsub, checkpoint_ev
     """rrule of L(d, rho, cd) = sum over cells (UTrans - target)^2 after nsub substeps.
 
     Returns (L, pullback) with pullback(Lbar) -> (NoTangent, d_bar, rho_bar, cd_bar);
-    the chunk is advanced by the forward run."""
+    the chunk is advanced by the forward run.  The first pullback runs the reverse sweep
+    and releases the adjoint's device buffers (and its hold on the chunk and field);
+    the gradients are linear in Lbar, so later calls scale the kept ones."""
     adj = Adjoint(ctx, chunk, field, nsub, checkpoint_every, p)
-    adj.forward(dt_fluid)
-    loss = adj.source_misfit(target)
+    try:
+        adj.forward(dt_fluid)
+        loss = adj.source_misfit(target)
+    except Exception:
+        adj.close()
+        raise
+    kept = {}
 
     def pullback(lbar=1.0):
-        adj.backward()
-        g = adj.gradients()
-        rho = None if g["rho"] is None else lbar * g["rho"]
-        return None, lbar * g["d"], rho, lbar * g["cd"]
+        if not kept:
+            try:
+                adj.backward()
+                kept.update(adj.gradients())
+            finally:
+                adj.close()
+        rho = None if kept["rho"] is None else lbar * kept["rho"]
+        return None, lbar * kept["d"], rho, lbar * kept["cd"]
 
     return loss, pullback
