@@ -235,15 +235,18 @@ int st_batch_info(const st_batch *batch, int32_t *total_items, int32_t *grid,
 
 /* Extension: autotuner of a batch.  Times every candidate (parcels per brick in
  * {1024 .. 16384}, block in {256, 512, 1024}, persistent workgroups per CU capped at
- * {all, 4, 3, 2}) over `trials` passes of one substep (or base->substeps_per_pass) and
- * restores the clouds and sources afterwards.  The result is cached by device and
+ * {all, 4, 3, 2}) as production runs it: one fluid step of 10 substeps (passes of
+ * base->substeps_per_pass) captured as a hipGraph, one warm-up replay, then
+ * max(trials, 5) timed replays; the candidate's figure is the median replay per pass.
+ * The clouds and sources are restored afterwards.  The result is cached by device and
  * workload in cache_path (JSON lines; NULL or "" disables the cache). */
 typedef struct st_tune_result {
     int32_t target_parcels_per_brick;
     int32_t block;
     int32_t blocks_per_cu;
     int32_t from_cache;  /* 1 if read from cache_path */
-    double ms_per_pass;
+    double ms_per_pass;            /* the winner's median replay per pass */
+    double runner_up_ms_per_pass;  /* the second-best candidate's */
 } st_tune_result;
 int st_autotune_batch(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *const *fields,
                       const st_params *base, double dt_fluid, int32_t trials, const char *cache_path,

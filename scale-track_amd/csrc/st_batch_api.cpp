@@ -36,6 +36,10 @@ struct st_batch {
     size_t lds = 0;
     double dt_sub = -1.0;            // substep the device table holds
     int pins = 0;                    // live graphs of this batch
+    st::FillItem *fills = nullptr;   // device: every field's src, then (owner flush) stray
+    uint32_t *fill_chunk0 = nullptr; // device [nfill + 1]
+    int32_t nfill = 0;
+    uint32_t fill_chunks = 0;
     bool pinned_members = false;     // the clouds and fields are pinned by this batch
 };
 
@@ -194,6 +198,8 @@ int destroy(st_batch *b) {
     if (b->item0) (void)hipFree(b->item0);
     if (b->targets) (void)hipFree(b->targets);
     if (b->counter) (void)hipFree(b->counter);
+    if (b->fills) (void)hipFree(b->fills);
+    if (b->fill_chunk0) (void)hipFree(b->fill_chunk0);
     st_ctx *ctx = b->ctx;
     delete b;
     release_ctx(ctx);
@@ -288,6 +294,24 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
     if (e == hipSuccess) e = hipMalloc((void **)&b->item0, (size_t)(n + 1) * sizeof(int32_t));
     if (e == hipSuccess) e = hipMalloc((void **)&b->targets, (size_t)n * sizeof(st::StatsTarget));
     if (e == hipSuccess) e = hipMalloc((void **)&b->counter, sizeof(unsigned int));
+    // the table that zeroes every field's source (and stray sums) in one launch
+    std::vector<st::FillItem> fills;
+    for (st_field *f : b->fields) fills.push_back({f->src, (size_t)f->ncell * 3});
+    if (p.flush == ST_FLUSH_OWNER)
+        for (st_field *f : b->fields) fills.push_back({f->stray, (size_t)f->ncell * 3});
+    std::vector<uint32_t> chunk0(fills.size() + 1, 0);
+    for (size_t i = 0; i < fills.size(); ++i) chunk0[i + 1] = chunk0[i] + st::fill_many_chunks(fills[i].n);
+    b->nfill = (int32_t)fills.size();
+    b->fill_chunks = chunk0.back();
+    if (e == hipSuccess) e = hipMalloc((void **)&b->fills, fills.size() * sizeof(st::FillItem));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->fill_chunk0, chunk0.size() * sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->fills, fills.data(), fills.size() * sizeof(st::FillItem), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->fill_chunk0, chunk0.data(), chunk0.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         destroy(b);
         return fail(ST_E_NOMEM, "batch allocation: %s", hipGetErrorString(e));
@@ -343,15 +367,9 @@ int st_batch_graph_capture(st_batch *b, double dt_fluid, int32_t nsub, int32_t z
         return fail(ST_E_DEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     }
     int r = ST_OK;
-    if (zero_source) {
-        for (st_field *f : b->fields) {
-            e = st::launch_fill_f64(f->src, 0.0, (size_t)f->ncell * 3, s);
-            if (e == hipSuccess && g->owner_flush) e = st::launch_fill_f64(f->stray, 0.0, (size_t)f->ncell * 3, s);
-            if (e != hipSuccess) {
-                r = fail(ST_E_DEVICE, "capture fill: %s", hipGetErrorString(e));
-                break;
-            }
-        }
+    if (zero_source) {  // every field's source (and stray sums) in one launch
+        e = st::launch_fill_many(b->fills, b->fill_chunk0, b->nfill, b->fill_chunks, s);
+        if (e != hipSuccess) r = fail(ST_E_DEVICE, "capture fill: %s", hipGetErrorString(e));
     }
     if (r == ST_OK) r = queue_passes(b, dt_sub, nsub, true);
     hipGraph_t graph = nullptr;

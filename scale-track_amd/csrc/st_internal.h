@@ -158,6 +158,15 @@ hipError_t launch_unpermute(const Soa<T> &in, const int32_t *pid, Soa<T> out, ui
                             hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s, const unsigned int *gate = nullptr);
 hipError_t launch_fill_f64(double *p, double value, size_t n, hipStream_t s);
+// Zeroes count arrays of doubles in one launch (graph-capturable).  chunk0[i] is the
+// first chunk of item i in chunks of fill_many_chunks(1) elements; chunk0[count] = chunks.
+struct FillItem {
+    double *p;
+    size_t n;
+};
+uint32_t fill_many_chunks(size_t n);
+hipError_t launch_fill_many(const FillItem *items, const uint32_t *chunk0, int32_t count, uint32_t chunks,
+                            hipStream_t s);
 hipError_t launch_iota(int32_t *p, uint32_t n, hipStream_t s);
 hipError_t launch_iota_from(int32_t *p, uint32_t n, int32_t base, hipStream_t s);
 // Tiles of the exclusive scan over n elements (entries of its tile-sum array).

@@ -884,6 +884,34 @@ __global__ void k_merge(double *src, double *stray, size_t n) {
     }
 }
 
+// Zeroes many arrays in one launch: workgroup w clears chunk w of the concatenation
+// (items ordered, chunk0[i] = first chunk of item i, chunk0[count] = all chunks).
+constexpr size_t kFillChunk = 16384;
+__global__ void __launch_bounds__(256) k_fill_many(const FillItem *items, const uint32_t *chunk0, int32_t count) {
+    const uint32_t w = blockIdx.x;
+    int32_t lo = 0, hi = count - 1;  // last item with chunk0 <= w
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (chunk0[mid] <= w)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const FillItem it = items[lo];
+    const size_t b0 = (size_t)(w - chunk0[lo]) * kFillChunk;
+    const size_t b1 = min(b0 + kFillChunk, it.n);
+    for (size_t i = b0 + threadIdx.x; i < b1; i += 256) it.p[i] = 0.0;
+}
+
+uint32_t fill_many_chunks(size_t n) { return (uint32_t)((n + kFillChunk - 1) / kFillChunk); }
+
+hipError_t launch_fill_many(const FillItem *items, const uint32_t *chunk0, int32_t count, uint32_t chunks,
+                            hipStream_t s) {
+    if (count <= 0 || chunks == 0) return hipSuccess;
+    k_fill_many<<<chunks, 256, 0, s>>>(items, chunk0, count);
+    return hipGetLastError();
+}
+
 static unsigned grid_for(size_t n, unsigned block) {
     size_t g = (n + block - 1) / block;
     if (g > 65536) g = 65536;

@@ -62,6 +62,7 @@ class st_tune_result(C.Structure):
         ("blocks_per_cu", C.c_int32),
         ("from_cache", C.c_int32),
         ("ms_per_pass", C.c_double),
+        ("runner_up_ms_per_pass", C.c_double),
     ]
 
 

@@ -397,7 +397,7 @@ class Batch:
             pass
 
 
-def autotune_batch(ctx, chunks, fields, p=None, dt_fluid=1e-3, trials=3, cache_path=None):
+def autotune_batch(ctx, chunks, fields, p=None, dt_fluid=1e-3, trials=5, cache_path=None):
     """st_autotune_batch: the fastest (target parcels per brick, block, workgroups per CU)
     for these clouds, timed on a snapshot that is restored afterwards; cached per device
     and workload in `cache_path` (JSON lines) when given."""
@@ -410,4 +410,5 @@ def autotune_batch(ctx, chunks, fields, p=None, dt_fluid=1e-3, trials=3, cache_p
                                 float(dt_fluid), int(trials),
                                 None if cache_path is None else str(cache_path).encode(), C.byref(r)))
     return {"target_parcels_per_brick": r.target_parcels_per_brick, "block": r.block,
-            "blocks_per_cu": r.blocks_per_cu, "ms_per_pass": r.ms_per_pass, "from_cache": bool(r.from_cache)}
+            "blocks_per_cu": r.blocks_per_cu, "ms_per_pass": r.ms_per_pass,
+            "runner_up_ms_per_pass": r.runner_up_ms_per_pass, "from_cache": bool(r.from_cache)}
