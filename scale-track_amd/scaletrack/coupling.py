@@ -287,6 +287,40 @@ class AsyncTracker:
         return path
 
 
+def borrow(tracker, name):
+    """The solver-side view of a tracker field: the pointer allocate_array_j hands out,
+    wrapped without a copy (juliaWrapper::reallocate swaps OpenFOAM's storage to it,
+    juliaWrapper.H:119-172).  Returns the (ncell, 3) array and the pointer, which
+    check_pointers compares against later."""
+    dt = np.dtype(tracker.scalar)
+    ptr = tracker.allocate_array_j(name, tracker.mesh.ncell, 3, dt.itemsize)
+    buf = (C.c_char * (tracker.mesh.ncell * 3 * dt.itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype=dt).reshape(tracker.mesh.ncell, 3), ptr
+
+
+def check_pointers(tracker, expected):
+    """juliaWrapper::checkPointers (juliaWrapper.H:78-97): the storage the solver borrowed
+    must still be the tracker's."""
+    for name, ptr in expected.items():
+        if getattr(tracker, name).ctypes.data != ptr:
+            raise RuntimeError(f"Pointer of field {name} was changed by OpenFOAM")
+
+
+def run_coupled(solver, tracker, dt, steps, per_volume=False, on_step=None):
+    """The solver's time loop of icoJuliaParcelFoam.C:118-186 with the tracker: each step
+    checks the borrowed pointers, calls evolve_cloud(dt) (the lag-one exchange), then
+    solves the fluid step with the momentum source UTrans / dt, writing U into the
+    tracker's storage.  `solver.U` must be the borrowed U (see borrow).  on_step(k, U)
+    is called with the velocity the parcel step k is about to be given."""
+    ptrs = {"U": tracker.U.ctypes.data, "UTrans": tracker.UTrans.ctypes.data}
+    for k in range(steps):
+        check_pointers(tracker, ptrs)
+        if on_step is not None:
+            on_step(k, tracker.U)
+        tracker.evolve_cloud(dt)
+        solver.step(dt, tracker.UTrans, per_volume=per_volume)
+
+
 def _sig4(x):
     """round(x, sigdigits=4) printed as Julia prints a Float32."""
     if x == 0 or not math.isfinite(x):
