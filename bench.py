@@ -63,7 +63,8 @@ def parse():
     ap.add_argument("--only", action="store_true",
                     help="time only the headline configuration (for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-parcels", type=int, default=1 << 22)
+    ap.add_argument("--cpu-parcels", type=int, default=1 << 23)
+    ap.add_argument("--cpu-parcels-serial", type=int, default=1 << 20)
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (JSON lines)")
     ap.add_argument("--workload", choices=["forward", "batch", "adjoint", "channel"], default="forward",
                     help="with --only: which workload to time (profiler runs)")
@@ -98,6 +99,7 @@ def run_variant(stk, ctx, case, precision, p, steps, warmup, keep_state=False):
     for _ in range(steps):
         field.zero_source()
         stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
+        field.merge_source()  # the step's source complete on the device, as the solver reads it
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     # pass durations of the timed steps, from the HIP events around their pass launches
@@ -379,27 +381,54 @@ def adjoint_workload(stk, inputs, ctx, np_parcels=1 << 20, ncell=64, nsub=200, K
                       "sort_s = brick sort of the freshly uploaded state"}
 
 
-def cpu_baseline(case, nparcels):
-    """The oracle (C++ FP64 restatement, test infrastructure) on the first `nparcels`
-    parcels of the same workload for one fluid step (nsub = 10 substeps), serial and
-    OpenMP over the host's cores."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, nparcels, nparcels_serial, calls=5, runs=3):
+    """The oracle (C++ FP64 restatement of the reference's step, test infrastructure) on
+    the first parcels of the same workload over the forward schedule: `calls` fluid steps
+    of nsub = 10 substeps with the source zeroed per step, after one untimed warm-up
+    step; the median of `runs` runs.  OpenMP over the host's cores on `nparcels`
+    parcels, and the reference's serial loop shape on `nparcels_serial`."""
     from oracle import st_oracle
 
-    st = [a[:nparcels].copy() for a in case.state()]
     ncores = len(os.sched_getaffinity(0))
     res = {}
-    for nt in (1, ncores):
-        s = [a.copy() for a in st]
-        t0 = time.perf_counter()
-        st_oracle.evolve(s, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
-                         boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d,
-                         rho_d_by_rho_c=case.rho_d_by_rho_c, nthreads=nt)
-        res[nt] = nparcels * case.nsub / (time.perf_counter() - t0)
-    sample = (f"{nparcels} of the 2^24 parcels, 128^3 cells, 1 fluid step = {case.nsub} substeps, "
-              "FP64 oracle")
-    return ({"value": res[ncores], "unit": "parcel-updates/s", "cores": ncores, "kind": "port",
-             "sample": sample},
-            {"value": res[1], "unit": "parcel-updates/s", "cores": 1, "kind": "port", "sample": sample})
+    for nt, npar in ((ncores, nparcels), (1, nparcels_serial)):
+        st0 = [a[:npar].copy() for a in case.state()]
+
+        def run(s):
+            for _ in range(calls):
+                st_oracle.evolve(s, case.U, case.n, case.origin, case.ending, case.dt_fluid, nsub=case.nsub,
+                                 boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d,
+                                 rho_d_by_rho_c=case.rho_d_by_rho_c, nthreads=nt)
+
+        st_oracle.evolve([a.copy() for a in st0], case.U, case.n, case.origin, case.ending, case.dt_fluid,
+                         nsub=case.nsub, boundary=case.boundary, mu_c=case.mu_c, rho_d=case.rho_d,
+                         rho_d_by_rho_c=case.rho_d_by_rho_c, nthreads=nt)  # warm-up
+        rates = []
+        for _ in range(runs):
+            s = [a.copy() for a in st0]
+            t0 = time.perf_counter()
+            run(s)
+            rates.append(npar * case.nsub * calls / (time.perf_counter() - t0))
+        res[nt] = (float(np.median(rates)), npar)
+    model = cpu_model()
+    out = []
+    for nt in (ncores, 1):
+        rate, npar = res[nt]
+        out.append({"value": rate, "unit": "parcel-updates/s", "cores": nt, "kind": "port",
+                    "sample": f"{npar} of the 2^24 parcels, 128^3 cells, {calls} fluid steps x {case.nsub} "
+                              f"substeps after one warm-up step, median of {runs} runs, FP64 oracle",
+                    "cpu_model": model})
+    return out[0], out[1]
 
 
 def main():
@@ -425,7 +454,7 @@ def main():
     copy_gbs = ctx.copy_bandwidth(1 << 30, reps=10)
     if args.only and args.workload == "batch":
         out = batch_workload(stk, inputs, ctx, steps=args.steps, warmup=args.warmup, tune_cache=args.tune_cache)
-        out["copy_kernel_GBs"] = copy_gbs
+        out["calibration_copy_GBs"] = copy_gbs
         print(json.dumps(out))
         return
     if args.only and args.workload == "adjoint":
@@ -491,7 +520,9 @@ def main():
             "kernel": "k_pass_sorted" if args.method == "sorted" else "k_pass_plain",
             "kernel_ms_per_pass": head["kernel_ms_per_pass"],
             "bytes_per_pass": bpp,
-            "copy_kernel_GBs": copy_gbs,
+            # the streaming copy the traffic counters are calibrated on (a known byte count),
+            # not a bandwidth ceiling
+            "calibration_copy_GBs": copy_gbs,
         },
         "stray_fraction": head["stray_fraction"],
     }
@@ -529,7 +560,7 @@ def main():
         line["adjoint_16M"] = adjoint_workload(stk, inputs, ctx, np_parcels=1 << 24, ncell=128, nsub=50, K=10)
         print(f"# adjoint: {json.dumps(line['adjoint'])}", file=sys.stderr, flush=True)
     if not args.no_cpu and not args.only:
-        omp, serial = cpu_baseline(case, min(args.cpu_parcels, case.np))
+        omp, serial = cpu_baseline(case, min(args.cpu_parcels, case.np), min(args.cpu_parcels_serial, case.np))
         line["cpu_baseline"] = omp
         line["cpu_baseline_serial"] = serial
     ctx.close()

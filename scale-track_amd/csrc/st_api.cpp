@@ -518,10 +518,14 @@ int ensure_part(st_cloud *c, int32_t nslots) {
 }
 
 int check_tile(const st_ctx *ctx, int prec, const int32_t brick[3]) {
-    const size_t need = prec == ST_FP64 ? tile_lds_bytes<double>(brick) : tile_lds_bytes<float>(brick);
+    // the halo tile plus the kernels' static LDS (statistics reduction, batch item slot)
+    const size_t need =
+        (prec == ST_FP64 ? tile_lds_bytes<double>(brick) : tile_lds_bytes<float>(brick)) + kPassStaticLds;
     if (need > ctx->max_lds)
-        return fail(ST_E_INVALID, "brick %dx%dx%d needs %zu B of LDS with its halo; this device allows %zu",
-                    brick[0], brick[1], brick[2], need, ctx->max_lds);
+        return fail(ST_E_INVALID,
+                    "brick %dx%dx%d needs %zu B of LDS with its halo (%zu B of it the kernel's own); this device "
+                    "allows %zu",
+                    brick[0], brick[1], brick[2], need, kPassStaticLds, ctx->max_lds);
     return ST_OK;
 }
 
@@ -1016,6 +1020,12 @@ int st_field_zero_source(st_field *f) {
     if (f->stray_dirty) ST_HIP(st::launch_fill_f64(f->stray, 0.0, (size_t)f->ncell * 3, f->ctx->stream));
     f->stray_dirty = false;
     return ST_OK;
+}
+
+int st_field_merge_source(st_field *f) {
+    if (!f) return fail(ST_E_INVALID, "null field");
+    ST_TRY(check_ctx(f->ctx));
+    return merge_source(f);
 }
 
 int st_field_download_source(st_field *f, void *out, int32_t elem_bytes) {

@@ -54,7 +54,7 @@ namespace {
 
 // Static LDS of the batch kernel (its statistics reduction and item slot), kept free
 // next to the tile when bricks are sized for a number of workgroups per CU.
-constexpr size_t kBatchStaticLds = 1024;
+constexpr size_t kBatchStaticLds = st::kPassStaticLds;
 
 // Brick of about `cells` cells: powers of two, x extent largest (rows stay contiguous),
 // at most `max_cells` including the one-cell halo.

@@ -97,6 +97,10 @@ __host__ __device__ inline double order_key_decode(unsigned long long k) {
     return c.d;
 }
 
+// Static LDS the pass kernels declare besides the tile, rounded up: the statistics
+// reduction's 7 x 16 doubles (896 B) and the batch kernel's item slot.
+constexpr size_t kPassStaticLds = 1024;
+
 // LDS bytes of the halo tile of a brick (velocities in T, sums in double).
 template <class T>
 inline size_t tile_lds_bytes(const int32_t bd[3]) {

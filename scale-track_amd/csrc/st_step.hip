@@ -344,6 +344,7 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
 template <class T>
 __device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, double *slot) {
     __shared__ double red[7][1024 / kWave];
+    static_assert(sizeof(red) + sizeof(int32_t) <= kPassStaticLds, "static LDS above the budget check_tile uses");
     double v[7];
     for (int c = 0; c < 3; ++c) {
         v[c] = (double)box.lo[c];

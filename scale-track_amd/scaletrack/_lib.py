@@ -99,6 +99,7 @@ SIGNATURES = {
     "st_field_destroy": (C.c_int, [_vp]),
     "st_field_upload_U": (C.c_int, [_vp, _vp, C.c_int32]),
     "st_field_zero_source": (C.c_int, [_vp]),
+    "st_field_merge_source": (C.c_int, [_vp]),
     "st_field_download_source": (C.c_int, [_vp, _vp, C.c_int32]),
     "st_evolve": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int32, C.POINTER(st_params),
                             C.POINTER(C.c_double)]),

@@ -159,6 +159,10 @@ class TwoWayEulerian:
     def zero_source(self):
         check(lib.st_field_zero_source(self.handle))
 
+    def merge_source(self):
+        """Complete UTrans on the device (st_field_merge_source), without a host copy."""
+        check(lib.st_field_merge_source(self.handle))
+
     def download_source(self, dtype=np.float64):
         out = np.empty((self.N, 3), dtype=dtype)
         check(lib.st_field_download_source(self.handle, _ptr(out), out.dtype.itemsize))

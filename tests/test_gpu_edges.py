@@ -253,3 +253,24 @@ def test_owner_flush_dense_and_sparse_bricks_accumulate(ctx, per, precision, bou
         assert np.array_equal(a, b), name
     chunk.close()
     field.close()
+
+
+def test_tile_lds_check_counts_static_lds(ctx):
+    """An FP64 32 x 8 x 8 brick: its halo tile (34 * 10 * 10 cells * 48 B = 163,200 B) fits
+    the 160 KiB a workgroup may use, but not with the kernel's own static LDS; the step
+    refuses it with ST_E_INVALID before launching, instead of failing on the device."""
+    from scaletrack import _lib
+
+    case = inputs.taylor_green_box(np_parcels=4096, ncell=32, boundary=1)
+    mesh = stk.Mesh(case.n, case.origin, case.ending)
+    field = stk.TwoWayEulerian(ctx, mesh, stk.FP64)
+    field.upload_U(case.U)
+    chunk = stk.Chunk(ctx, case.np, case.mu_c, case.rho_d, case.rho_d_by_rho_c, stk.FP64)
+    chunk.set_state(*case.state())
+    chunk.upload()
+    with pytest.raises(_lib.ScaleTrackError) as ei:
+        stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, stk.params(boundary=1, brick=(32, 8, 8)))
+    assert ei.value.status == _lib.ST_E_INVALID and "LDS" in str(ei.value)
+    stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, stk.params(boundary=1, brick=(32, 8, 4)))
+    chunk.close()
+    field.close()
