@@ -65,12 +65,14 @@ def test_cavity_100_fluid_steps(ctx, method):
     field.close()
 
 
-@pytest.mark.parametrize("precision", [stk.FP64, stk.FP32])
-def test_taylor_green_16M(ctx, precision):
+@pytest.mark.parametrize("precision,passes_per_launch", [(stk.FP64, 1), (stk.FP32, 1), (stk.FP64, 10), (stk.FP32, 10)])
+def test_taylor_green_16M(ctx, precision, passes_per_launch):
+    """The headline workload, one fluid step of ten one-substep passes, as ten launches
+    or as one launch of the per-XCD work queue."""
     dt = np.float64 if precision == stk.FP64 else np.float32
     case = inputs.taylor_green_box(np_parcels=1 << 24, ncell=128, dtype=dt, boundary=stk.PERIODIC)
     chunk, field = device_cloud(ctx, case, precision)
-    p = stk.params(boundary=stk.PERIODIC, substeps_per_pass=1)
+    p = stk.params(boundary=stk.PERIODIC, substeps_per_pass=1, passes_per_launch=passes_per_launch)
     field.zero_source()
     stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
     src = field.download_source()
