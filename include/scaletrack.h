@@ -97,10 +97,9 @@ typedef struct st_params {
                                   cloud when the share of parcels outside their brick in the
                                   step's last pass exceeds this (decided on the device, also
                                   inside captured graphs); default 0.08 */
-    int32_t passes_per_launch; /* sorted method: passes run by one persistent launch that
-                                  takes (pass, brick) items from a work queue per XCD, a
-                                  brick's pass starting once its previous pass is done;
-                                  1 = one launch per pass (default) */
+    int32_t passes_per_launch; /* sorted method: passes run by one persistent launch, each
+                                  workgroup owning the same bricks in every pass; 1 = one
+                                  launch per pass (default) */
 } st_params;
 
 /* ---- library ---------------------------------------------------------------- */
@@ -113,11 +112,6 @@ int st_device_count(int32_t *count);
 int st_ctx_create(int32_t device, st_ctx **out);
 int st_ctx_destroy(st_ctx *ctx);
 int st_ctx_synchronize(st_ctx *ctx);
-/* No counterpart in the reference.  Work queues the multi-pass launch
- * (st_params.passes_per_launch > 1) keeps on this context's device, one per XCD: 0 when
- * the device's workgroups do not report XCD ids the library can use, in which case
- * such steps run one launch per pass (same results). */
-int st_ctx_queues(const st_ctx *ctx, int32_t *queues);
 
 /* ---- parcel cloud: the reference's Chunk (:329-364) ------------------------- */
 /* n parcels; mu_c, rho_d and rho_d_by_rho_c are the chunk scalars of

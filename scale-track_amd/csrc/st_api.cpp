@@ -224,18 +224,13 @@ static bool same_geometry(const st_cloud *c, const st_field *f, const st_params 
 
 // The sort buffers for nbr bricks and the cloud's capacity: grown (when may_alloc) or
 // only checked (ST_E_STATE when too small).
-// The multi-pass launch's queue holds nbr bricks (or is not used on this device).
-static bool queue_fits(const st_cloud *c, int64_t nbr) {
-    return c->ctx->xccs == 0 || (c->queue && c->queue_cap >= st::queue_words(c->ctx->xccs, nbr));
-}
-
 static int sort_buffers(st_cloud *c, int64_t nbr, bool may_alloc) {
     const int32_t rows = sort_rows(c->n, nbr);
     const size_t n1 = (size_t)(c->cap > 0 ? c->cap : 1);
     if (!may_alloc) {
         const bool ok = c->counts && c->cursor && c->brick_start && c->bins_cap >= nbr && c->keys &&
                         c->keys_cap >= n1 && (rows == 0 || (c->hist && c->hist_cap >= (size_t)rows * (size_t)nbr)) &&
-                        c->buf[1 - c->cur][0] && c->pid[1 - c->cur] && queue_fits(c, nbr);
+                        c->buf[1 - c->cur][0] && c->pid[1 - c->cur];
         return ok ? ST_OK
                   : fail(ST_E_STATE, "the cloud's sort buffers do not hold its %lld parcels; a sort (st_cloud_sort or "
                                      "st_evolve) sizes them",
@@ -249,7 +244,6 @@ static int sort_buffers(st_cloud *c, int64_t nbr, bool may_alloc) {
     c->bins_cap = (int32_t)std::max<int64_t>(c->bins_cap, nbr);
     ST_TRY(grow(c, &c->keys, c->keys_cap, n1, "sort keys"));
     if (rows > 0) ST_TRY(grow(c, &c->hist, c->hist_cap, (size_t)rows * (size_t)nbr, "sort histograms"));
-    if (c->ctx->xccs > 0) ST_TRY(grow(c, &c->queue, c->queue_cap, st::queue_words(c->ctx->xccs, nbr), "work queue"));
     return alloc_set(c, 1 - c->cur);
 }
 
@@ -335,14 +329,20 @@ int range_check(st_cloud *c, st_field *f) {
     return ST_OK;
 }
 
-// Persistent workgroups of the multi-pass launch: as many as are resident at once
-// (the kernel's occupancy per CU on every CU); they take items from the queues until
-// none are left.
+// Persistent workgroups of the multi-pass launch: the most that fill every CU equally
+// (a multiple of the CU count, at most the kernel's occupancy per CU) and split the
+// bricks evenly, so no workgroup and no CU owns more bricks than another; the most that
+// fit when no such count exists.
 template <class T>
 static int multi_grid(const st_ctx *ctx, const st_cloud *c, const st_params &p, int32_t tile_cells) {
     const size_t lds = (size_t)tile_cells * 3 * (sizeof(double) + sizeof(T));
     const int per_cu = pass_multi_blocks_per_cu<T>(p.boundary, c->has_rho, p.block, lds);
-    return per_cu < 1 ? 0 : per_cu * ctx->cus;
+    if (per_cu < 1) return 0;
+    for (int m = per_cu; m >= 1; --m) {
+        const int64_t g = (int64_t)m * ctx->cus;
+        if (g <= c->nbricks && c->nbricks % g == 0) return (int)g;
+    }
+    return (int)std::min<int64_t>((int64_t)per_cu * ctx->cus, std::max<int32_t>(1, c->nbricks));
 }
 
 int32_t tail_blocks(const st_cloud *c, const st_params &p) {
@@ -361,7 +361,6 @@ int32_t stats_slots(const st_cloud *c, const st_params &p) {
 static void observe_step(st_cloud *c) {
     if (!c->strays_pending || hipEventQuery(c->strays_ev) != hipSuccess) return;
     c->strays_pending = false;
-    if (c->strays_host[2]) c->queue_fault = true;
     c->last_stray_fraction = c->strays_den > 0 ? (double)c->strays_host[0] / (double)c->strays_den : 0.0;
     if (c->strays_host[1] && c->sorted) c->sorted_n = std::max(c->sorted_n, c->strays_den);
 }
@@ -404,14 +403,10 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     const int32_t tblocks = tail_blocks(c, p);
     PassArgs<T> ta = a;
     ta.tail0 = c->brick_start + c->nbricks;
-    // passes of one persistent launch (sorted method, no checkpoint, no tail, and a
-    // device whose XCDs the queue can tell apart; otherwise one launch per pass)
-    const bool queued = p.method == ST_METHOD_SORTED && !ck && tblocks == 0 && p.passes_per_launch > 1 &&
-                        ctx->xccs > 0 && queue_fits(c, c->nbricks);
-    const int group = queued ? p.passes_per_launch : 1;
+    // passes of one persistent launch (sorted method, no checkpoint, no tail)
+    const int group = p.method == ST_METHOD_SORTED && !ck && tblocks == 0 ? std::max(1, p.passes_per_launch) : 1;
     const int grid = group > 1 ? multi_grid<T>(ctx, c, p, a.tile_cells) : 0;
     if (group > 1 && grid < 1) return fail(ST_E_RANGE, "the multi-pass kernel does not fit: block %d", p.block);
-    const PassQueue pq{c->queue, ctx->xccs, 1u << 18};  // a wait gives up after about half a second
 
     // Statistics (bounding box, strays) are taken in the last pass only.
     const int32_t nslots = stats_slots(c, p);
@@ -430,7 +425,7 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
         a.part = next >= nsub ? sb.part : nullptr;
         a.ck = done == 0 ? ck : nullptr;
         if (k > 1)
-            ST_HIP(launch_pass_multi<T>(a, k, pq, bnd, p.block, grid, s));
+            ST_HIP(launch_pass_multi<T>(a, k, bnd, p.block, grid, s));
         else if (p.method == ST_METHOD_SORTED)
             ST_HIP(launch_pass_sorted<T>(a, bnd, p.block, s));
         else
@@ -468,8 +463,6 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
         // the gate this step's re-sort ran on, or a device word that is always 0 (the
         // host never writes the pinned words: an earlier copy may still be in flight)
         ST_HIP(hipMemcpyAsync(c->strays_host + 1, resort ? sb.gate : c->resort + 2, 4, hipMemcpyDeviceToHost, s));
-        ST_HIP(hipMemcpyAsync(c->strays_host + 2, group > 1 ? c->queue + kQueueStride * ctx->xccs : c->resort + 2, 4,
-                              hipMemcpyDeviceToHost, s));
         ST_HIP(hipEventRecord(c->strays_ev, s));
         c->strays_pending = true;
         c->strays_den = c->n;
@@ -493,9 +486,6 @@ int prepare(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int nsub, co
     if (c->prec != f->prec) return fail(ST_E_INVALID, "cloud and field precision differ");
     if (!c->uploaded) return fail(ST_E_STATE, "cloud has no parcels uploaded");
     if (!f->U_set) return fail(ST_E_STATE, "field velocity U was never uploaded");
-    if (c->queue_fault)
-        return fail(ST_E_DEVICE, "a multi-pass launch of this cloud gave up waiting for a brick's previous pass; the cloud's parcels and "
-                    "the step's source are invalid (upload the parcels again)");
     if (c->fresh) ST_TRY(range_check(c, f));
     return ST_OK;
 }
@@ -604,34 +594,6 @@ int st_device_count(int32_t *count) {
     return ST_OK;
 }
 
-// The number of XCDs the multi-pass launch may keep one queue each for: the device's
-// XCD count, provided the workgroups of a launch as large as that kernel's report XCD
-// ids 0 .. count - 1 and every one of them; 0 otherwise.
-static int probe_xccs(st_ctx *c) {
-    int nx = 0;
-    if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, c->device) != hipSuccess || nx < 1 ||
-        nx > st::kMaxQueues)
-        return 0;
-    const uint32_t n = (uint32_t)(8 * c->cus);
-    uint32_t *d = nullptr;
-    if (hipMalloc((void **)&d, n * sizeof(uint32_t)) != hipSuccess) return 0;
-    std::vector<uint32_t> h(n, 0xffffffffu);
-    hipError_t e = st::launch_fill_u32(d, 0xffffffffu, n, c->stream);
-    if (e == hipSuccess) e = st::launch_xcc_probe(d, n, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(d);
-    if (e != hipSuccess) return 0;
-    std::vector<int> seen(nx, 0);
-    for (uint32_t v : h) {
-        if (v >= (uint32_t)nx) return 0;
-        seen[v] = 1;
-    }
-    for (int v : seen)
-        if (!v) return 0;
-    return nx;
-}
-
 int st_ctx_create(int32_t device, st_ctx **out) {
     if (!out) return fail(ST_E_INVALID, "null output handle");
     *out = nullptr;
@@ -655,9 +617,6 @@ int st_ctx_create(int32_t device, st_ctx **out) {
         lds > 0)
         c->cus = lds;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    // the multi-pass launch's queues (a failed check only turns those launches into one
-    // launch per pass)
-    if (e == hipSuccess) c->xccs = probe_xccs(c);
     // st_evolve_noalloc's scratch address on this device (a failed lookup only disables
     // that entry point, which then says so)
     if (e == hipSuccess) (void)noalloc_setup(device);
@@ -670,12 +629,6 @@ int st_ctx_create(int32_t device, st_ctx **out) {
         return fail(ST_E_DEVICE, "context creation: %s", hipGetErrorString(e));
     }
     *out = c;
-    return ST_OK;
-}
-
-int st_ctx_queues(const st_ctx *ctx, int32_t *queues) {
-    if (!ctx || !queues) return fail(ST_E_INVALID, "null argument");
-    *queues = ctx->xccs;
     return ST_OK;
 }
 
@@ -730,7 +683,7 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->bad, 4);
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->resort, 4 * sizeof(unsigned int));
     if (r == ST_OK && e == hipSuccess) e = st::launch_fill_u32(c->resort, 0u, 4, ctx->stream);
-    if (r == ST_OK && e == hipSuccess) e = hipHostMalloc((void **)&c->strays_host, 16, hipHostMallocDefault);
+    if (r == ST_OK && e == hipSuccess) e = hipHostMalloc((void **)&c->strays_host, 8, hipHostMallocDefault);
     if (r == ST_OK && e == hipSuccess) e = hipEventCreateWithFlags(&c->strays_ev, hipEventDisableTiming);
     if (r == ST_OK && e == hipSuccess) e = st::launch_iota(c->pid[0], (uint32_t)n, ctx->stream);
     if (r != ST_OK || e != hipSuccess) {
@@ -758,7 +711,6 @@ int st_cloud_destroy(st_cloud *c) {
     if (c->cursor) hipFree(c->cursor);
     if (c->hist) hipFree(c->hist);
     if (c->brick_start) hipFree(c->brick_start);
-    if (c->queue) hipFree(c->queue);
     if (c->bbox) hipFree(c->bbox);
     if (c->part) hipFree(c->part);
     if (c->strays) hipFree(c->strays);
@@ -801,7 +753,6 @@ int st_cloud_upload(st_cloud *c, const void *d, const void *x, const void *y, co
     c->permuted = false;
     c->sorted = false;
     c->strays_pending = false;
-    c->queue_fault = false;
     ++c->epoch;
     return ST_OK;
 }
@@ -827,10 +778,6 @@ int st_cloud_download(st_cloud *c, void *d, void *x, void *y, void *z, void *u, 
     for (int f = 0; f < 7; ++f)
         if (bytes) ST_HIP(hipMemcpyAsync(dst[f], c->buf[from][f], bytes, hipMemcpyDeviceToHost, s));
     ST_HIP(hipStreamSynchronize(s));
-    observe_step(c);
-    if (c->queue_fault)
-        return fail(ST_E_DEVICE, "a multi-pass launch of this cloud gave up waiting for a brick's previous pass; the cloud's parcels and "
-                    "the step's source are invalid (upload the parcels again)");
     return ST_OK;
 }
 

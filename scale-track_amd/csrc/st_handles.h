@@ -47,10 +47,6 @@ struct st_ctx {
     size_t max_lds = 65536;      // LDS bytes one workgroup may use on this device
     size_t lds_per_cu = 163840;  // LDS bytes of one CU (all resident workgroups together)
     int cus = 256;               // compute units of the device
-    // XCDs whose workgroups report their own ids 0 .. xccs - 1 (checked by a probe at
-    // creation): the multi-pass launch keeps one work queue per XCD.  0 when unchecked
-    // or not so: multi-pass launches then run as one launch per pass.
-    int xccs = 0;
     // Clouds, fields, graphs, batches and adjoints hold a reference: st_ctx_destroy only
     // marks the context closed while any of them is alive; the last release frees it.
     int refs = 0;
@@ -81,8 +77,6 @@ struct st_cloud {
     size_t hist_cap = 0;                 // elements
     int32_t *brick_start = nullptr;
     int32_t bins_cap = 0;
-    uint32_t *queue = nullptr;           // the multi-pass launch's work queue (st::queue_words)
-    size_t queue_cap = 0;                // words
     bool sorted = false;
     int32_t sort_n[3] = {0, 0, 0}, sort_bd[3] = {0, 0, 0}, sort_periodic = -1;
     int32_t nb[3] = {0, 0, 0};
@@ -94,9 +88,7 @@ struct st_cloud {
     int32_t part_cap = 0;
     unsigned int *strays = nullptr;      // device
     unsigned int *bad = nullptr;         // device
-    unsigned int *strays_host = nullptr; // pinned: [0] strays of the last step, [1] its re-sort gate,
-                                         // [2] its multi-pass queue's error word
-    bool queue_fault = false;            // a multi-pass wait gave up: that step's results are wrong
+    unsigned int *strays_host = nullptr; // pinned: [0] strays of the last step, [1] its re-sort gate
     hipEvent_t strays_ev = nullptr;
     bool strays_pending = false;
     int64_t strays_den = 0;  // parcels behind strays_host

@@ -101,20 +101,6 @@ __host__ __device__ inline double order_key_decode(unsigned long long k) {
 // reduction's 7 x 16 doubles (896 B) and the batch kernel's item slot.
 constexpr size_t kPassStaticLds = 1024;
 
-// The work queue of the multi-pass launch, in one device array of 32-bit words:
-//   [kQueueStride * q]            claim counter of queue q (q < nq), one 128-byte line each
-//   [kQueueStride * nq]           error word: nonzero once a wait gave up
-//   [kQueueStride * (nq + 1) + b] passes of brick b completed and published
-// Every word starts at zero before each launch.
-constexpr int kQueueStride = 32;
-inline size_t queue_words(int nq, int64_t nbricks) { return (size_t)kQueueStride * (size_t)(nq + 1) + (size_t)nbricks; }
-constexpr int kMaxQueues = 16;
-struct PassQueue {
-    uint32_t *words;
-    int32_t nq;     // queues: one per XCD, brick b in queue b % nq
-    uint32_t spin;  // polls a wait makes before it gives up and sets the error word
-};
-
 // LDS bytes of the halo tile of a brick (velocities in T, sums in double).
 template <class T>
 inline size_t tile_lds_bytes(const int32_t bd[3]) {
@@ -131,15 +117,11 @@ hipError_t launch_pass_sorted(const PassArgs<T> &a, int bnd, int block, hipStrea
 template <class T>
 hipError_t launch_pass_plain(const PassArgs<T> &a, int bnd, int block, hipStream_t s, uint32_t grid = 0);
 // npass passes of a.nsteps substeps each in one persistent launch of `grid` workgroups
-// taking (pass, brick) items from the queue q.words (zeroed first, on the stream);
-// a.part is filled in the last pass.
+// (each owning the same bricks in every pass); a.part is filled in the last pass.
 template <class T>
-hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, const PassQueue &q, int bnd, int block, int grid,
-                             hipStream_t s);
+hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, int bnd, int block, int grid, hipStream_t s);
 template <class T>
 int pass_multi_blocks_per_cu(int bnd, int rho, int block, size_t lds);
-// XCD (accelerator complex) of each of n workgroups of one launch, written to out[0..n).
-hipError_t launch_xcc_probe(uint32_t *out, uint32_t n, hipStream_t s);
 // Persistent pass over clouds [0, ncloud) of descs: work items item0[0] .. item0[ncloud]-1
 // (counter must hold item0[0] at launch); rho: every one of these clouds carries a
 // per-parcel density array (else none does).

@@ -344,7 +344,7 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
 template <class T>
 __device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, double *slot) {
     __shared__ double red[7][1024 / kWave];
-    static_assert(sizeof(red) + 64 <= kPassStaticLds, "static LDS above the budget check_tile uses");
+    static_assert(sizeof(red) + sizeof(int32_t) <= kPassStaticLds, "static LDS above the budget check_tile uses");
     double v[7];
     for (int c = 0; c < 3; ++c) {
         v[c] = (double)box.lo[c];
@@ -430,21 +430,11 @@ struct TileShape {
     }
 };
 
-// Calls process_brick makes in every thread of the workgroup: staged() once the tile is
-// in LDS (after its barrier), stepped() once every parcel is stepped (after the barrier
-// that ends the parcel loop, before the flush).  With kDrain, every thread has waited for
-// all of its earlier memory operations by the time stepped() runs, empty bricks included.
-struct NoHooks {
-    static constexpr bool kDrain = false;
-    __device__ __forceinline__ void staged() const {}
-    __device__ __forceinline__ void stepped() const {}
-};
-
 // One brick of a sorted cloud: stage its velocities in LDS, step its parcels, add its
 // source sums to HBM once per cell.  Every thread of the workgroup must call it.
-template <class T, int BND, bool RHO, bool CKPT = false, class Hooks = NoHooks>
+template <class T, int BND, bool RHO, bool CKPT = false>
 __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t brick, int32_t nsteps,
-                                              double *lds, double *part, const Hooks &hooks = Hooks{}) {
+                                              double *lds, double *part) {
     const int32_t bi = brick % a.nb[0];
     const int32_t bj = (brick / a.nb[0]) % a.nb[1];
     const int32_t bk = brick / (a.nb[0] * a.nb[1]);
@@ -476,7 +466,7 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     env.Ut = Ut;
 
     // the brick ranges do not change during a launch: read them as constants (scalar
-    // loads), so no vector-memory wait is spent on them
+    // loads), so no vector-memory wait is spent on them in a persistent launch either
     typedef __attribute__((address_space(4))) const int32_t ConstI32;
     const ConstI32 *bs_k = (const ConstI32 *)a.brick_start;
     const int32_t p0 = bs_k[brick], p1 = bs_k[brick + 1];
@@ -547,7 +537,6 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
         }
     }
     __syncthreads();
-    hooks.staged();
 
     Box<T> box;
     box.init();
@@ -597,11 +586,8 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
                 many(B, A);
             }
         }
-    } else if (Hooks::kDrain) {
-        __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
-    hooks.stepped();
 
     // Add the tile's sums to HBM, kStage cells per thread at a time with every load of
     // the read-modify-writes issued before the first add.  With the owner flush only
@@ -667,220 +653,36 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeo
     process_brick<T, BND, RHO, CKPT>(a, blockIdx.x, a.nsteps, lds, a.part);
 }
 
-// A generic pointer known to address global memory, cast through the global address
-// space so that accesses through it compile to global (not flat) instructions.
-template <class U>
-__device__ __forceinline__ U *as_global(U *p) {
-    return (U *)(__attribute__((address_space(1))) U *)(uintptr_t)p;
-}
-
-// The XCD (accelerator complex, with its own L2) the calling wave runs on.
-__device__ __forceinline__ uint32_t xcc_id() {
-    uint32_t v;
-    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-    return v & 15u;
-}
-
-// The queue state of a k_pass_multi workgroup, in LDS (no register holds it across a
-// brick's parcel loop).  Thread 0 alone writes it; every thread reads item, next and
-// ready after the barrier that ends an item.
-struct QueueSlots {
-    int32_t item;       // the item to step next
-    int32_t next;       // the item after it (claimed)
-    uint32_t ready;     // done[] of item's brick as read ahead
-    int32_t claim;      // the item after next, claimed during the current item
-    uint32_t ahead;     // done[] of next's brick, read during the current item
-    int32_t prev;       // brick stepped last and not yet published (-1: none)
-    uint32_t prev_done; // the value to publish for it
-    int32_t cur;        // the brick being stepped, and the value it will publish
-    uint32_t cur_done;
-};
-
-// What k_pass_multi does inside process_brick.  staged(): thread 0 parks the item it
-// claimed and the done[] value it read at the item's start (both have completed with
-// the brick's loads).  stepped(): every thread has waited for its stores of the previous
-// brick, so thread 0 publishes it, makes the current brick the one to publish next, and
-// advances the workgroup's items.
-struct QueueHooks {
-    static constexpr bool kDrain = true;
-    int32_t claim;
-    uint32_t ahead;
-    uint32_t *done;
-    QueueSlots *qs;
-    __device__ __forceinline__ void staged() const {
-        if (threadIdx.x == 0) {
-            qs->claim = claim;
-            qs->ahead = ahead;
-        }
-    }
-    __device__ __forceinline__ void stepped() const {
-        if (threadIdx.x == 0) {
-            if (qs->prev >= 0)
-                __hip_atomic_store(done + qs->prev, qs->prev_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            qs->prev = qs->cur;
-            qs->prev_done = qs->cur_done;
-            qs->item = qs->next;
-            qs->next = qs->claim;
-            qs->ready = qs->ahead;
-        }
-    }
-};
-
-// Several passes of one sorted cloud in one persistent launch, as a work queue of
-// (pass, brick) items.  Item (k, b) steps brick b's parcels through pass k; it needs
-// (k - 1, b) done -- the brick's parcels, and its own cells of `src` (owner flush), are
-// written by the brick's previous pass alone -- and nothing else: halo and stray sums go
-// to `stray` with atomics, and `U` is frozen.  So no pass waits for the end of the one
-// before, and the launch has one ramp and one drain instead of one per pass.
-//
-// Each L2 cache serves one XCD, and L2 caches are not coherent with each other during a
-// launch.  So brick b is always stepped on the same XCD: queue q = b % nq holds bricks
-// q, q + nq, ... pass by pass, and a workgroup takes items from the queue of the XCD it
-// runs on (the host checks beforehand that workgroups report XCD ids 0 .. nq - 1).
-// Within one XCD the L2 is the point of coherence: a brick's stores are in L2 once they
-// have completed, and before its first load of an item every wave invalidates the
-// CU's vector L1 at agent scope (buffer_inv sc1; the workgroup-scope form leaves the L1
-// of a workgroup that runs on one CU as it is).
-//
-// Per queue, items are claimed from one counter in order, two ahead; a dependency
-// always has a smaller index.  At an item's start thread 0 claims the item after next
-// and reads done[] of the next item's brick: both complete under this brick's loads,
-// and the next item decides from that value whether it must wait.  A finished item is
-// published (done[b] = k + 1) after the next item's parcel loop, where every thread has
-// waited for its memory operations anyway; a workgroup about to wait publishes its
-// finished item first.  So the unpublished item of smallest index never waits on
-// another, and every wait ends.  A value read ahead can only be too small (the counters
-// grow during a launch), which only sends the workgroup to the coherent poll.
-// A poll gives up after q.spin tries, or at once when the error word is already set,
-// and sets it (results are then wrong and the host reports it), so every workgroup
-// exits, after one time-out at most.  `part` (statistics) is filled in the last pass
-// only.
+// Several passes of one sorted cloud in one persistent launch.  Workgroup w owns bricks
+// w, w + gridDim.x, ... in every pass and runs its bricks pass by pass, so a brick's
+// pass k + 1 follows its pass k in the same workgroup: the parcels of a brick, and its
+// own cells of `src` (owner flush), are read and written by that workgroup alone, with
+// its barriers ordering them, and everything shared (halo and stray sums) goes to
+// `stray` with atomics.  Pass-major order keeps each pass a full sweep over the
+// parcel arrays in HBM.  `part` (statistics) is filled in the last pass only.
 template <class T, int BND, bool RHO>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4)))
-k_pass_multi(PassArgs<T> a, int32_t npass, PassQueue pq) {
+k_pass_multi(PassArgs<T> a, int32_t npass) {
     extern __shared__ double lds[];
-    __shared__ QueueSlots qs;
-    const int32_t nq = pq.nq;
-    const int32_t q = (int32_t)(xcc_id() % (uint32_t)nq);
-    const int32_t nbq = (a.nbricks - q + nq - 1) / nq;
-    const int32_t total = nbq * npass;
-    uint32_t *ctr = pq.words + kQueueStride * q;
-    uint32_t *err = pq.words + kQueueStride * nq;
-    uint32_t *done = err + kQueueStride;
-
-    if (threadIdx.x == 0) {
-        qs.item = (int32_t)atomicAdd(ctr, 1u);
-        qs.next = (int32_t)atomicAdd(ctr, 1u);
-        qs.ready = 0u;
-        qs.prev = -1;
+    for (int32_t k = 0; k < npass; ++k) {
+        double *part = k == npass - 1 ? a.part : nullptr;
+        for (int32_t brick = blockIdx.x; brick < a.nbricks; brick += gridDim.x) {
+            process_brick<T, BND, RHO>(a, brick, a.nsteps, lds, part);
+            __syncthreads();  // the flush's tile reads are done before the next staging
+        }
     }
-    __syncthreads();
-    for (;;) {
-        const int32_t item = __builtin_amdgcn_readfirstlane(qs.item);
-        if (item >= total) break;
-        const uint32_t ready = __builtin_amdgcn_readfirstlane(qs.ready);
-        const int32_t k = item / nbq;
-        const int32_t b = q + (item - k * nbq) * nq;
-        int32_t claim = 0;
-        uint32_t ahead = 0;
-        if (threadIdx.x == 0) {
-            claim = (int32_t)atomicAdd(ctr, 1u);
-            const int32_t nx = qs.next;
-            if (nx < total) {
-                const int32_t kk = nx / nbq;
-                ahead = __hip_atomic_load(done + q + (nx - kk * nbq) * nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            qs.cur = b;
-            qs.cur_done = (uint32_t)k + 1u;
-        }
-        if (k > 0 && ready < (uint32_t)k) {
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                if (qs.prev >= 0)
-                    __hip_atomic_store(done + qs.prev, qs.prev_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                qs.prev = -1;
-                // once any wait has given up, none waits any more: the launch then ends
-                // after at most one time-out
-                for (uint32_t n = 0; __hip_atomic_load(done + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (uint32_t)k;) {
-                    if (++n >= pq.spin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        atomicOr(err, 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(8);
-                }
-            }
-            __syncthreads();
-        }
-        asm volatile("buffer_inv sc1" ::: "memory");
-        const QueueHooks hooks{claim, ahead, done, &qs};
-        // The arguments are read again for every item, through a pointer the compiler
-        // cannot see through: hoisted out of the loop, every value derived from them
-        // would stay live across the whole launch, and the registers they take are the
-        // parcel loop's (it would spill).  `a` is the first kernel argument, at offset 0
-        // of the kernel-argument segment.
-        uint64_t ka = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
-        asm volatile("" : "+s"(ka));
-        PassArgs<T> ai;
-        static_assert(sizeof(PassArgs<T>) % 4 == 0, "argument block copied in 32-bit words");
-        {
-            const __attribute__((address_space(4))) uint32_t *src = (const __attribute__((address_space(4))) uint32_t *)ka;
-            uint32_t w[sizeof(PassArgs<T>) / 4];
-#pragma unroll
-            for (size_t i = 0; i < sizeof(PassArgs<T>) / 4; ++i) w[i] = src[i];
-            __builtin_memcpy(&ai, w, sizeof(ai));
-            // the pointers came through plain words: mark them global again, or every
-            // access through them would be a flat one
-            ai.s.d = as_global(ai.s.d);
-            ai.s.x = as_global(ai.s.x);
-            ai.s.y = as_global(ai.s.y);
-            ai.s.z = as_global(ai.s.z);
-            ai.s.u = as_global(ai.s.u);
-            ai.s.v = as_global(ai.s.v);
-            ai.s.w = as_global(ai.s.w);
-            ai.s.rho = as_global(ai.s.rho);
-            ai.U = as_global(ai.U);
-            ai.src = as_global(ai.src);
-            ai.stray = as_global(ai.stray);
-            ai.brick_start = as_global(ai.brick_start);
-            ai.part = as_global(ai.part);
-            ai.ck = as_global(ai.ck);
-            ai.tail0 = as_global(ai.tail0);
-        }
-        process_brick<T, BND, RHO, false, QueueHooks>(ai, b, ai.nsteps, lds, k == npass - 1 ? ai.part : nullptr, hooks);
-        __syncthreads();  // the item slots are advanced; the flush's tile reads are done
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0 && qs.prev >= 0)
-        __hip_atomic_store(done + qs.prev, qs.prev_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class T>
-hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, const PassQueue &q, int bnd, int block, int grid,
-                             hipStream_t s) {
+hipError_t launch_pass_multi(const PassArgs<T> &a, int32_t npass, int bnd, int block, int grid, hipStream_t s) {
     const size_t lds = (size_t)a.tile_cells * 3 * (sizeof(double) + sizeof(T));
-    hipError_t e = launch_fill_u32(q.words, 0u, queue_words(q.nq, a.nbricks), s);
-    if (e != hipSuccess) return e;
     with_forms(bnd, a.s.rho != nullptr, [&](auto B, auto R) {
-        k_pass_multi<T, decltype(B)::value, decltype(R)::value><<<grid, block, lds, s>>>(a, npass, q);
+        k_pass_multi<T, decltype(B)::value, decltype(R)::value><<<grid, block, lds, s>>>(a, npass);
     });
     return hipGetLastError();
 }
-template hipError_t launch_pass_multi<float>(const PassArgs<float> &, int32_t, const PassQueue &, int, int, int,
-                                             hipStream_t);
-template hipError_t launch_pass_multi<double>(const PassArgs<double> &, int32_t, const PassQueue &, int, int, int,
-                                              hipStream_t);
-
-__global__ void k_xcc_probe(uint32_t *out, uint32_t n) {
-    if (threadIdx.x == 0 && blockIdx.x < n) out[blockIdx.x] = xcc_id();
-}
-
-hipError_t launch_xcc_probe(uint32_t *out, uint32_t n, hipStream_t s) {
-    if (!n) return hipSuccess;
-    k_xcc_probe<<<n, 64, 0, s>>>(out, n);
-    return hipGetLastError();
-}
+template hipError_t launch_pass_multi<float>(const PassArgs<float> &, int32_t, int, int, int, hipStream_t);
+template hipError_t launch_pass_multi<double>(const PassArgs<double> &, int32_t, int, int, int, hipStream_t);
 
 template <class T>
 int pass_multi_blocks_per_cu(int bnd, int rho, int block, size_t lds) {

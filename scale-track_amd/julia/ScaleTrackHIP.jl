@@ -105,11 +105,6 @@ function Context(device::Integer=0)
     Context(h[])
 end
 synchronize(ctx::Context) = check(ccall((:st_ctx_synchronize, lib), Cint, (Ptr{Cvoid},), ctx.h))
-function queues(ctx::Context)
-    n = Ref{Int32}(0)
-    check(ccall((:st_ctx_queues, lib), Cint, (Ptr{Cvoid}, Ref{Int32}), ctx.h, n))
-    Int(n[])
-end
 function destroy!(ctx::Context)
     ctx.h == C_NULL || check(ccall((:st_ctx_destroy, lib), Cint, (Ptr{Cvoid},), ctx.h))
     ctx.h = C_NULL

@@ -86,7 +86,6 @@ SIGNATURES = {
     "st_ctx_create": (C.c_int, [C.c_int32, _pp]),
     "st_ctx_destroy": (C.c_int, [_vp]),
     "st_ctx_synchronize": (C.c_int, [_vp]),
-    "st_ctx_queues": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "st_cloud_create": (C.c_int, [_vp, C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_int32, _pp]),
     "st_cloud_destroy": (C.c_int, [_vp]),

@@ -48,13 +48,6 @@ class Context:
     def synchronize(self):
         check(lib.st_ctx_synchronize(self.handle))
 
-    def queues(self):
-        """Per-XCD work queues of the multi-pass launch on this device (0: such steps run
-        one launch per pass)."""
-        n = C.c_int32()
-        check(lib.st_ctx_queues(self.handle, C.byref(n)))
-        return n.value
-
     def last_kernel_ms(self):
         ms, n = C.c_double(), C.c_int32()
         check(lib.st_last_kernel_ms(self.handle, C.byref(ms), C.byref(n)))
