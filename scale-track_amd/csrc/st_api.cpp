@@ -424,16 +424,17 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
         const int next = done + k * a.nsteps;
         a.part = next >= nsub ? sb.part : nullptr;
         a.ck = done == 0 ? ck : nullptr;
-        if (k > 1)
+        if (k > 1) {
             ST_HIP(launch_pass_multi<T>(a, k, bnd, p.block, grid, s));
-        else if (p.method == ST_METHOD_SORTED)
-            ST_HIP(launch_pass_sorted<T>(a, bnd, p.block, s));
-        else
-            ST_HIP(launch_pass_plain<T>(a, bnd, p.block, s));
-        if (tblocks > 0) {  // after the sorted launch: its owner flush is done
+        } else if (tblocks > 0) {  // the bricks and the unsorted tail in one launch
             ta.nsteps = a.nsteps;
-            ta.part = a.part ? a.part + (size_t)c->nbricks * kStatSlot : nullptr;
-            ST_HIP(launch_pass_plain<T>(ta, bnd, p.block, s, (uint32_t)tblocks));
+            ta.part = a.part;
+            ta.ck = a.ck;
+            ST_HIP(launch_pass_sorted_tail<T>(ta, (uint32_t)tblocks, bnd, p.block, s));
+        } else if (p.method == ST_METHOD_SORTED) {
+            ST_HIP(launch_pass_sorted<T>(a, bnd, p.block, s));
+        } else {
+            ST_HIP(launch_pass_plain<T>(a, bnd, p.block, s));
         }
         passes += k;
         done = next;

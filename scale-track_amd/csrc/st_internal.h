@@ -116,6 +116,10 @@ template <class T>
 hipError_t launch_pass_sorted(const PassArgs<T> &a, int bnd, int block, hipStream_t s);
 template <class T>
 hipError_t launch_pass_plain(const PassArgs<T> &a, int bnd, int block, hipStream_t s, uint32_t grid = 0);
+// A sorted pass and the pass over the cloud's unsorted tail (a.tail0 set, tail_blocks
+// workgroups of `block` parcels) in one launch; a.part holds nbricks + tail_blocks slots.
+template <class T>
+hipError_t launch_pass_sorted_tail(const PassArgs<T> &a, uint32_t tail_blocks, int bnd, int block, hipStream_t s);
 // npass passes of a.nsteps substeps each in one persistent launch of `grid` workgroups
 // (each owning the same bricks in every pass); a.part is filled in the last pass.
 template <class T>

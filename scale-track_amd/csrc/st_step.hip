@@ -375,15 +375,17 @@ __device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, double
     }
 }
 
-// One thread per parcel, from parcel 0 or, over a sorted cloud's unsorted tail, from
-// tail0[0] on (each live tail parcel counts as outside its brick in the statistics).
+// One workgroup of the plain method: one thread per parcel, parcels base + blk *
+// blockDim.x + threadIdx.x, deposits with FP64 atomics into `out`, statistics into
+// `slot` (when statistics are taken).  Over a sorted cloud's unsorted tail (a.tail0
+// set) each live parcel counts as outside its brick.
 template <class T, int BND, bool RHO>
-__global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
-    const uint32_t base = a.tail0 ? (uint32_t)a.tail0[0] : 0u;
-    const uint32_t p = base + blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void plain_block(const PassArgs<T> &a, uint32_t base, uint32_t blk, double *out,
+                                            double *slot) {
+    const uint32_t p = base + blk * blockDim.x + threadIdx.x;
     Box<T> box;
     box.init();
-    GlobalEnv<T> env{a.U, a.src};
+    GlobalEnv<T> env{a.U, out};
     uint32_t tail = 0;
     if (p < a.np) {
         ParcelIn<T> in;
@@ -394,7 +396,15 @@ __global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
         else
             step_parcel<T, BND, RHO>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
     }
-    if (a.part) flush_stats(box, tail, a.part + (size_t)blockIdx.x * kStatSlot);
+    if (slot) flush_stats(box, tail, slot);
+}
+
+// One thread per parcel, from parcel 0 or, over a sorted cloud's unsorted tail, from
+// tail0[0] on.
+template <class T, int BND, bool RHO>
+__global__ void __launch_bounds__(1024) k_pass_plain(PassArgs<T> a) {
+    const uint32_t base = a.tail0 ? (uint32_t)a.tail0[0] : 0u;
+    plain_block<T, BND, RHO>(a, base, blockIdx.x, a.src, a.part ? a.part + (size_t)blockIdx.x * kStatSlot : nullptr);
 }
 
 // The adjoint's checkpoint of a parcel (the state it starts a segment with), written
@@ -653,6 +663,27 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeo
     process_brick<T, BND, RHO, CKPT>(a, blockIdx.x, a.nsteps, lds, a.part);
 }
 
+// A pass over a sorted cloud with an unsorted tail (parcels injected since its last
+// sort) in one launch: workgroups 0 .. nbricks - 1 step the bricks, the ones after them
+// the tail, one thread per parcel from a.tail0[0] on.  The tail's workgroups come last
+// in dispatch order, so they fill the slots the bricks leave while the launch drains.
+// Tail deposits go where deposits from outside a brick go -- `stray` with atomics under
+// the owner flush (bricks update their own cells of `src` with plain read-modify-writes
+// in the same launch), `src` with atomics under the atomic flush.  Statistics slots:
+// one per brick, then one per tail workgroup.
+template <class T, int BND, bool RHO>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4)))
+k_pass_sorted_tail(PassArgs<T> a) {
+    extern __shared__ double lds[];
+    if ((int32_t)blockIdx.x < a.nbricks) {
+        process_brick<T, BND, RHO, false>(a, blockIdx.x, a.nsteps, lds, a.part);
+    } else {
+        plain_block<T, BND, RHO>(a, (uint32_t)a.tail0[0], blockIdx.x - (uint32_t)a.nbricks,
+                                 a.flush == 0 ? a.stray : a.src,
+                                 a.part ? a.part + (size_t)blockIdx.x * kStatSlot : nullptr);
+    }
+}
+
 // Several passes of one sorted cloud in one persistent launch.  Workgroup w owns bricks
 // w, w + gridDim.x, ... in every pass and runs its bricks pass by pass, so a brick's
 // pass k + 1 follows its pass k in the same workgroup: the parcels of a brick, and its
@@ -801,6 +832,18 @@ hipError_t launch_pass_sorted(const PassArgs<T> &a, int bnd, int block, hipStrea
     });
     return hipGetLastError();
 }
+
+template <class T>
+hipError_t launch_pass_sorted_tail(const PassArgs<T> &a, uint32_t tail_blocks, int bnd, int block, hipStream_t s) {
+    const size_t lds = (size_t)a.tile_cells * 3 * (sizeof(double) + sizeof(T));
+    const uint32_t grid = (uint32_t)a.nbricks + tail_blocks;
+    with_forms(bnd, a.s.rho != nullptr, [&](auto B, auto R) {
+        k_pass_sorted_tail<T, decltype(B)::value, decltype(R)::value><<<grid, block, lds, s>>>(a);
+    });
+    return hipGetLastError();
+}
+template hipError_t launch_pass_sorted_tail<float>(const PassArgs<float> &, uint32_t, int, int, hipStream_t);
+template hipError_t launch_pass_sorted_tail<double>(const PassArgs<double> &, uint32_t, int, int, hipStream_t);
 
 template hipError_t launch_pass_plain<float>(const PassArgs<float> &, int, int, hipStream_t, uint32_t);
 template hipError_t launch_pass_plain<double>(const PassArgs<double> &, int, int, hipStream_t, uint32_t);
