@@ -136,7 +136,7 @@ int st_cloud_count(const st_cloud *cloud, int64_t *n);
  * precision; rho as in st_cloud_upload) after the current ones: they take the next
  * upload-order indices, so st_cloud_count grows by k and downloads return them last.
  * A sorted cloud stays sorted: sorted steps advance the injected parcels as an unsorted
- * tail (one extra launch behind each pass, in caller order) until a re-sort the device
+ * tail (workgroups after the bricks in each pass's launch, one thread per parcel) until a re-sort the device
  * decides on (params->resort_fraction) takes them into the bricks.  st_cloud_compact
  * drops the parcels an open boundary removed (ST_CHANNEL; d < 0), keeping the others
  * in their upload order with their indices renumbered from 0, and returns their number

@@ -94,9 +94,10 @@ struct st_cloud {
     int64_t strays_den = 0;  // parcels behind strays_host
     // Parcels [0, sorted_n) were in brick order after the last sort the host knows of;
     // parcels injected since then form an unsorted tail [brick_start[nbricks], n) that
-    // each pass steps with a plain launch.  The device re-sorts fold the tail in; the
-    // host learns of one when it reads the gate back, so sorted_n may lag (the tail
-    // launch reads its start from the device and covers n - sorted_n parcels at most).
+    // each pass steps in workgroups after the bricks.  The device re-sorts fold the tail
+    // in; the host learns of one when it reads the gate back, so sorted_n may lag (the
+    // tail's workgroups read its start from the device and cover n - sorted_n parcels at
+    // most).
     int64_t sorted_n = 0;
     double last_stray_fraction = -1.0;
     // device: [0] gate of the re-sort queued behind the last step (set by the statistics
@@ -171,7 +172,7 @@ int cloud_stats(st_cloud *c, int32_t nslots, StatsBuf &sb);
 // Statistics slots one step of the cloud uses with these params (sorted: one per brick,
 // then one per workgroup of the unsorted tail; plain: one per workgroup).
 int32_t stats_slots(const st_cloud *c, const st_params &p);
-// Workgroups of the tail launch of a sorted step (0: no tail).
+// Workgroups over the unsorted tail in a sorted step's launches (0: no tail).
 int32_t tail_blocks(const st_cloud *c, const st_params &p);
 // Whether the sort buffers hold the cloud's current parcel count and brick geometry.
 bool sort_buffers_fit(const st_cloud *c);

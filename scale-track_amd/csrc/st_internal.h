@@ -80,7 +80,7 @@ struct PassArgs {
     int32_t nsteps;
     int32_t flush;   // 0 owner, 1 atomic
     uint32_t np;
-    // plain launches over a sorted cloud's unsorted tail: the first parcel is read from
+    // steps over a sorted cloud's unsorted tail: the first parcel is read from
     // here (the brick ranges' end, brick_start[nbricks]); nullptr otherwise
     const int32_t *tail0;
 };

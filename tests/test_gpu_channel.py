@@ -117,9 +117,10 @@ def test_inject_and_compact(ctx, precision):
 @pytest.mark.parametrize("precision", [stk.FP64, stk.FP32])
 @pytest.mark.parametrize("resort_fraction", [0.02, 1.0])
 def test_injected_tail(ctx, precision, resort_fraction):
-    """Parcels injected into a sorted cloud are stepped as its unsorted tail (a plain
-    launch behind each sorted pass) until a re-sort decided on the device takes them in
-    (fraction 0.02: within a few steps; 1.0: never, the tail grows for the whole run).
+    """Parcels injected into a sorted cloud are stepped as its unsorted tail (one thread
+    per parcel, in workgroups after the bricks in each pass) until a re-sort decided on
+    the device takes them in (fraction 0.02: within a few steps; 1.0: never, the tail
+    grows for the whole run).
     Steps are queued back to back; every step's parcels and the summed source equal the
     oracle's.  A graph captured with a tail and the no-allocation step take part too."""
     dt = np.float64 if precision == stk.FP64 else np.float32
