@@ -115,18 +115,21 @@ def test_inject_and_compact(ctx, precision):
 
 
 @pytest.mark.parametrize("precision", [stk.FP64, stk.FP32])
-@pytest.mark.parametrize("resort_fraction", [0.02, 1.0])
-def test_injected_tail(ctx, precision, resort_fraction):
+@pytest.mark.parametrize("resort_fraction,flush", [(0.02, stk.FLUSH_OWNER), (1.0, stk.FLUSH_OWNER),
+                                                   (1.0, stk.FLUSH_ATOMIC)])
+def test_injected_tail(ctx, precision, resort_fraction, flush):
     """Parcels injected into a sorted cloud are stepped as its unsorted tail (one thread
     per parcel, in workgroups after the bricks in each pass) until a re-sort decided on
     the device takes them in (fraction 0.02: within a few steps; 1.0: never, the tail
-    grows for the whole run).
+    grows for the whole run), under the owner flush (tail deposits into the stray array)
+    and the atomic flush (tail deposits into the source).
     Steps are queued back to back; every step's parcels and the summed source equal the
     oracle's.  A graph captured with a tail and the no-allocation step take part too."""
     dt = np.float64 if precision == stk.FP64 else np.float32
     case = inputs.channel_case(np_parcels=20000, n=(32, 16, 16), length=(1.0, 1.0, 1.0), dtype=dt, dt_fluid=1e-2)
     chunk, field = device(ctx, case, precision)
-    p = stk.params(boundary=stk.CHANNEL, substeps_per_pass=1, brick=(8, 4, 4), resort_fraction=resort_fraction)
+    p = stk.params(boundary=stk.CHANNEL, substeps_per_pass=1, brick=(8, 4, 4), resort_fraction=resort_fraction,
+                   flush=flush)
     st = case.state()
     src = np.zeros((case.ncell, 3))
     from scaletrack import _lib

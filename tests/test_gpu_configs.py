@@ -9,7 +9,8 @@ records decisions):
   one-substep passes, FP64 and the FP32 form: state bit-exact, source within 1e-12,
   and the momentum balance sum(S) = sum(m (v_start - v_end)) over the whole cloud;
 * 64 uneven clouds (1e3 .. 2e6 parcels on 32^3 .. 96^3 cells) in one persistent batch,
-  FP32 form, one fluid step: every cloud bit-exact and within 1e-12;
+  FP32 form, one fluid step: every cloud bit-exact and within 1e-12; the same clouds
+  autotuned, captured as a hipGraph and replayed twice;
 * adjoint of the source misfit at 2^20 parcels on 64^3 cells over 200 substeps with a
   checkpoint every 20: directional derivative against the oracle's dual numbers
   (1e-9) and against central differences with eps = 1e-6 and frozen decisions (1e-4).
@@ -106,6 +107,43 @@ def test_batch_64_clouds(ctx):
         for name, a, b in zip("dxyzuvw", ch.download(), ref):
             assert np.array_equal(a, b), f"cloud {i}: {name} differs"
         assert_source(f.download_source(), ref_src, f"cloud {i}")
+    batch.close()
+    for ch, f in objs:
+        ch.close()
+        f.close()
+
+
+def test_batch_64_clouds_tuned_graph(ctx, tmp_path):
+    """The batch as production runs it: the autotuner picks the brick target, block and
+    workgroups per CU for the 64 clouds (restoring them afterwards), the batch is built
+    with its choice, one fluid step (every UTrans zeroed, 10 one-substep passes) is
+    captured as a hipGraph and replayed twice, with the sources and parcels downloaded
+    between the replays.  After each replay every cloud's source is the oracle's for
+    that fluid step (1e-12), and after the second every parcel is bit-exact."""
+    cases = inputs.batch_cases(64, dtype=np.float32)
+    objs = [device_cloud(ctx, c, stk.FP32) for c in cases]
+    chunks, fields = [o[0] for o in objs], [o[1] for o in objs]
+    p = stk.params(boundary=stk.PERIODIC, substeps_per_pass=1)
+    tuned = stk.autotune_batch(ctx, chunks, fields, p, dt_fluid=cases[0].dt_fluid, trials=5,
+                               cache_path=tmp_path / "tune.jsonl")
+    assert tuned["ms_per_pass"] > 0 and tuned["runner_up_ms_per_pass"] >= tuned["ms_per_pass"]
+    batch = stk.Batch(ctx, chunks, fields, stk.params(boundary=stk.PERIODIC, substeps_per_pass=1, block=tuned["block"]),
+                      target_parcels_per_brick=tuned["target_parcels_per_brick"], blocks_per_cu=tuned["blocks_per_cu"])
+    g = batch.graph(cases[0].dt_fluid, cases[0].nsub, zero_source=True)
+    states = [c.state() for c in cases]
+    for replay in range(2):
+        g.launch()
+        for i, (c, (ch, f)) in enumerate(zip(cases, objs)):
+            ref_src, _ = st_oracle.evolve(states[i], c.U, c.n, c.origin, c.ending, c.dt_fluid, nsub=c.nsub,
+                                          precision=st_oracle.FP32, boundary=c.boundary, mu_c=c.mu_c, rho_d=c.rho_d,
+                                          rho_d_by_rho_c=c.rho_d_by_rho_c, rho_c=c.rho_c, rho=c.rho,
+                                          substeps_per_pass=1, nthreads=THREADS)
+            assert_source(f.download_source(), ref_src, f"replay {replay}, cloud {i}")
+            got = ch.download()
+            if replay == 1:
+                for name, a, b in zip("dxyzuvw", got, states[i]):
+                    assert np.array_equal(a, b), f"cloud {i}: {name} differs after two replays"
+    g.close()
     batch.close()
     for ch, f in objs:
         ch.close()
