@@ -307,6 +307,30 @@ bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p) {
     return true;
 }
 
+// Parcels among (x, y, z)[0 .. k) outside the mesh the cloud is sorted for, by the
+// device's range check (k_range_check).
+template <class T>
+int64_t cells_outside(const st_cloud *c, const void *x, const void *y, const void *z, int64_t k) {
+    st_mesh m;
+    for (int a = 0; a < 3; ++a) {
+        m.n[a] = c->sort_n[a];
+        m.origin[a] = c->sort_o[a];
+        m.ending[a] = c->sort_e[a];
+    }
+    const Geo g = make_geo(m);
+    const T *p[3] = {(const T *)x, (const T *)y, (const T *)z};
+    int64_t bad = 0;
+    for (int64_t i = 0; i < k; ++i) {
+        bool out = false;
+        for (int a = 0; a < 3; ++a) {
+            const double v = std::floor(((double)p[a][i] - g.o[a]) * g.rd[a]);
+            out |= !(v >= 0.0 && v < (double)g.n[a]);
+        }
+        bad += out;
+    }
+    return bad;
+}
+
 int range_check(st_cloud *c, st_field *f) {
     hipStream_t s = c->ctx->stream;
     ST_HIP(launch_fill_u32(c->bad, 0u, 1, s));
@@ -510,11 +534,14 @@ int ensure_part(st_cloud *c, int32_t nslots) {
                     "the cloud's statistics slots would be reallocated (%d -> %d), but %d graph(s), batch(es) "
                     "or adjoint(s) hold them; destroy them first",
                     c->part_cap, nslots, c->pins);
+    // grown by half at least: a cloud whose unsorted tail grows step by step (injection)
+    // needs a few more slots every step, and each reallocation waits for the device
+    const int32_t cap = (int32_t)std::min<int64_t>(INT32_MAX, std::max<int64_t>(nslots, (int64_t)c->part_cap * 3 / 2));
     if (c->part) ST_HIP(hipFree(c->part));
     c->part = nullptr;
     c->part_cap = 0;
-    ST_HIP(hipMalloc((void **)&c->part, (size_t)nslots * kStatSlot * sizeof(double)));
-    c->part_cap = nslots;
+    ST_HIP(hipMalloc((void **)&c->part, (size_t)cap * kStatSlot * sizeof(double)));
+    c->part_cap = cap;
     return ST_OK;
 }
 
@@ -712,6 +739,9 @@ int st_cloud_destroy(st_cloud *c) {
     if (c->cursor) hipFree(c->cursor);
     if (c->hist) hipFree(c->hist);
     if (c->brick_start) hipFree(c->brick_start);
+    if (c->inj_buf) hipFree(c->inj_buf);
+    if (c->stage) hipHostFree(c->stage);
+    if (c->stage_ev) hipEventDestroy(c->stage_ev);
     if (c->bbox) hipFree(c->bbox);
     if (c->part) hipFree(c->part);
     if (c->strays) hipFree(c->strays);
@@ -854,18 +884,49 @@ int st_cloud_inject(st_cloud *c, int64_t k, const void *d, const void *x, const 
     }
     if (!positive_all(d, k, c->es)) return fail(ST_E_INVALID, "injected diameters must be > 0");
     if (c->has_rho && !positive_all(rho, k, c->es)) return fail(ST_E_INVALID, "injected densities must be > 0");
+    // A sorted cloud knows its mesh: the injected parcels are checked against it here, on
+    // the host, as the first step after an upload checks on the device (floor((x - origin)
+    // * rDelta) in [0, N) per axis, in the same double arithmetic), so that no step has
+    // to wait for a check.
+    const bool checked = c->sorted;
+    if (checked) {
+        const int64_t bad = c->es == 8 ? cells_outside<double>(c, x, y, z, k) : cells_outside<float>(c, x, y, z, k);
+        if (bad)
+            return fail(ST_E_RANGE,
+                        "%lld injected parcels lie outside the mesh [origin, ending): their cell index "
+                        "floor((x - origin) / delta) is outside [0, N) on some axis",
+                        (long long)bad);
+    }
     const int64_t n1 = c->n + k;
     if (n1 > c->cap) ST_TRY(st_cloud_reserve(c, std::max<int64_t>(n1, c->cap + c->cap / 2)));
     hipStream_t s = c->ctx->stream;
-    for (int f = 0; f < kNF; ++f) {
-        if (f == 7 && !c->has_rho) continue;
-        ST_HIP(hipMemcpyAsync((char *)c->buf[c->cur][f] + (size_t)c->n * c->es, src[f], (size_t)k * c->es,
-                              hipMemcpyHostToDevice, s));
+    // The host arrays go through a pinned staging buffer of the cloud: copied into it on
+    // the host, while the device may still be stepping earlier work, then read by a
+    // kernel queued on the stream; the call returns without waiting for the stream.
+    const int nf = c->has_rho ? kNF : 7;
+    const size_t part = (size_t)k * c->es;
+    if (!c->stage_ev) ST_HIP(hipEventCreateWithFlags(&c->stage_ev, hipEventDisableTiming));
+    else ST_HIP(hipEventSynchronize(c->stage_ev));  // the last copy out of the staging buffer is done
+    if (part * nf > c->stage_cap) {
+        if (c->stage) ST_HIP(hipHostFree(c->stage));
+        c->stage = nullptr;
+        c->stage_cap = 0;
+        if (hipHostMalloc(&c->stage, part * nf, hipHostMallocDefault) != hipSuccess)
+            return fail(ST_E_NOMEM, "cannot stage %lld injected parcels in pinned host memory", (long long)k);
+        c->stage_cap = part * nf;
     }
-    ST_HIP(st::launch_iota_from(c->pid[c->cur] + c->n, (uint32_t)k, (int32_t)c->n, s));
-    ST_HIP(hipStreamSynchronize(s));
-    if (!c->fresh) c->fresh_from = c->n;
-    c->fresh = true;
+    for (int f = 0; f < nf; ++f) std::memcpy((char *)c->stage + (size_t)f * part, src[f], part);
+    void *dstage = nullptr;
+    ST_HIP(hipHostGetDevicePointer(&dstage, c->stage, 0));
+    ST_TRY(copy_injected(c, dstage, c->n, k));
+    // in a sorted cloud the new parcels are its unsorted tail: ordered by cell, the tail's
+    // waves can sum their deposits before the atomics
+    ST_TRY(sort_injected(c, c->n, k));
+    ST_HIP(hipEventRecord(c->stage_ev, s));
+    if (!checked) {
+        if (!c->fresh) c->fresh_from = c->n;
+        c->fresh = true;
+    }
     c->n = n1;
     c->uploaded = true;
     // the new parcels sit at the end: sorted steps step them as the unsorted tail until

@@ -77,6 +77,11 @@ struct st_cloud {
     size_t hist_cap = 0;                 // elements
     int32_t *brick_start = nullptr;
     int32_t bins_cap = 0;
+    void *inj_buf = nullptr;             // keys, indices and radix-sort scratch of sort_injected
+    size_t inj_cap = 0;                  // bytes
+    void *stage = nullptr;               // pinned host staging of injected parcels
+    size_t stage_cap = 0;                // bytes
+    hipEvent_t stage_ev = nullptr;       // recorded behind the last copy out of `stage`
     bool sorted = false;
     int32_t sort_n[3] = {0, 0, 0}, sort_bd[3] = {0, 0, 0}, sort_periodic = -1;
     int32_t nb[3] = {0, 0, 0};
@@ -153,6 +158,15 @@ void release_ctx(st_ctx *ctx);
 // Makes the context's device current and waits for its stream (destroy paths).
 void quiesce_ctx(st_ctx *ctx);
 int alloc_set(st_cloud *c, int which);
+// Orders parcels [n0, n0 + k) of a sorted cloud by cell of its sorted mesh (parcels just
+// injected: they form its unsorted tail); queued on the context stream (st_inject.hip).
+int sort_injected(st_cloud *c, int64_t n0, int64_t k);
+// Queues the copy of k injected parcels from pinned, device-mapped host memory (nf
+// arrays of k values: d x y z u v w [rho]) into the cloud at [n0, n0 + k), with their
+// upload-order indices n0 .. n0 + k - 1 (st_inject.hip).
+int copy_injected(st_cloud *c, const void *stage, int64_t n0, int64_t k);
+// The kernels' geometry of a uniform mesh (origin, 1/delta, counts).
+Geo make_geo(const st_mesh &m);
 int validate_params(const st_params &p);
 bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p);
 int sort_any(st_cloud *c, st_field *f, const st_params &p);

@@ -375,26 +375,92 @@ __device__ __forceinline__ void flush_stats(Box<T> &box, uint32_t strays, double
     }
 }
 
+// The plain method's environment for a pass of one substep, where a parcel deposits
+// exactly once (at the end of the pass): the deposit is kept, for a wave-wide
+// reduction after the step, instead of being added at once.
+template <class T>
+struct DeferEnv : GlobalEnv<T> {
+    uint32_t cell = 0xffffffffu;  // global cell index of the deposit, or none
+    double s[3] = {0.0, 0.0, 0.0};
+    __device__ __forceinline__ void deposit(const Cell &cl, const typename GlobalEnv<T>::Slot &, double a, double b,
+                                            double c) {
+        cell = cl.c;
+        s[0] = a;
+        s[1] = b;
+        s[2] = c;
+    }
+};
+
+// Adds one deposit per lane (cell 0xffffffff: none) into `out` with FP64 atomics after a
+// segmented reduction across the wave: lanes holding the same cell as their neighbours
+// (parcels ordered by cell, as injected parcels are) are summed first, and only the
+// first lane of each run of equal cells adds, so a run of r lanes costs 3 atomics
+// instead of 3 r.  The runs are found with ballots and summed with a suffix sum by
+// doubling, each lane adding the lane d above it while that lane is in its run.  When
+// no two neighbouring lanes share a cell, every lane adds its own deposit.  Every lane
+// of the wave must call it.
+__device__ __forceinline__ void wave_deposit(double *out, uint32_t cell, double a, double b, double c) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const uint32_t up = __shfl_up(cell, 1, kWave);
+    const bool head = lane == 0 || up != cell;
+    const unsigned long long runs = __ballot(!head && cell != 0xffffffffu);
+    if (runs != 0ull) {
+        const uint32_t down = __shfl_down(cell, 1, kWave);
+        const unsigned long long tails = __ballot(lane == kWave - 1 || down != cell);
+        const int end = lane + __builtin_ctzll(tails >> lane);  // last lane of this lane's run
+        for (int d = 1; d < kWave; d <<= 1) {
+            const double ta = __shfl_down(a, d, kWave);
+            const double tb = __shfl_down(b, d, kWave);
+            const double tc = __shfl_down(c, d, kWave);
+            if (lane + d <= end) {
+                a += ta;
+                b += tb;
+                c += tc;
+            }
+        }
+    }
+    if (head && cell != 0xffffffffu) {
+        double *q = out + 3u * cell;
+        atomic_add_global(q, a);
+        atomic_add_global(q + 1, b);
+        atomic_add_global(q + 2, c);
+    }
+}
+
 // One workgroup of the plain method: one thread per parcel, parcels base + blk *
 // blockDim.x + threadIdx.x, deposits with FP64 atomics into `out`, statistics into
 // `slot` (when statistics are taken).  Over a sorted cloud's unsorted tail (a.tail0
-// set) each live parcel counts as outside its brick.
-template <class T, int BND, bool RHO>
+// set) each live parcel counts as outside its brick.  With WAVE_SUMS and one substep
+// per pass, each wave reduces its deposits before the atomics (wave_deposit).
+template <class T, int BND, bool RHO, bool WAVE_SUMS = false>
 __device__ __forceinline__ void plain_block(const PassArgs<T> &a, uint32_t base, uint32_t blk, double *out,
                                             double *slot) {
     const uint32_t p = base + blk * blockDim.x + threadIdx.x;
     Box<T> box;
     box.init();
-    GlobalEnv<T> env{a.U, out};
     uint32_t tail = 0;
-    if (p < a.np) {
-        ParcelIn<T> in;
-        load_parcel<T, RHO>(a.s, p, a.ph.rho, in);
-        if (a.tail0 && in.d > (T)0) tail = 1;
-        if (a.nsteps == 1)
+    if (WAVE_SUMS && a.nsteps == 1) {
+        DeferEnv<T> env;
+        env.U = a.U;
+        env.src = out;
+        if (p < a.np) {
+            ParcelIn<T> in;
+            load_parcel<T, RHO>(a.s, p, a.ph.rho, in);
+            if (a.tail0 && in.d > (T)0) tail = 1;
             step_parcel_one<T, BND, RHO>(a.g, a.ph, a.s, in, p, env, box, true, NoPrefetch{});
-        else
-            step_parcel<T, BND, RHO>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
+        }
+        wave_deposit(out, env.cell, env.s[0], env.s[1], env.s[2]);
+    } else {
+        GlobalEnv<T> env{a.U, out};
+        if (p < a.np) {
+            ParcelIn<T> in;
+            load_parcel<T, RHO>(a.s, p, a.ph.rho, in);
+            if (a.tail0 && in.d > (T)0) tail = 1;
+            if (a.nsteps == 1)
+                step_parcel_one<T, BND, RHO>(a.g, a.ph, a.s, in, p, env, box, true, NoPrefetch{});
+            else
+                step_parcel<T, BND, RHO>(a.g, a.ph, a.s, in, p, a.nsteps, env, box, NoPrefetch{});
+        }
     }
     if (slot) flush_stats(box, tail, slot);
 }
@@ -678,8 +744,8 @@ k_pass_sorted_tail(PassArgs<T> a) {
     if ((int32_t)blockIdx.x < a.nbricks) {
         process_brick<T, BND, RHO, false>(a, blockIdx.x, a.nsteps, lds, a.part);
     } else {
-        plain_block<T, BND, RHO>(a, (uint32_t)a.tail0[0], blockIdx.x - (uint32_t)a.nbricks,
-                                 a.flush == 0 ? a.stray : a.src,
+        plain_block<T, BND, RHO, true>(a, (uint32_t)a.tail0[0], blockIdx.x - (uint32_t)a.nbricks,
+                                       a.flush == 0 ? a.stray : a.src,
                                  a.part ? a.part + (size_t)blockIdx.x * kStatSlot : nullptr);
     }
 }

@@ -236,3 +236,36 @@ def test_channel_full_shape_fp32(ctx):
 
 def test_channel_full_shape_fp64_walls(ctx):
     _full(ctx, stk.FP64, stk.WALL)
+
+
+@pytest.mark.parametrize("precision", [stk.FP64, stk.FP32])
+def test_injected_parcels_outside_the_mesh(ctx, precision):
+    """A parcel injected outside the mesh (here exactly on the outlet plane, whose cell
+    index is N) is refused: by st_cloud_inject itself when the cloud is sorted (it
+    knows its mesh; nothing is appended), else by the next step's check on the device,
+    as for an upload."""
+    dt = np.float64 if precision == stk.FP64 else np.float32
+    case = inputs.channel_case(np_parcels=2000, n=(16, 8, 8), length=(0.5, 1.0, 1.0), dtype=dt, dt_fluid=1e-2)
+    p = stk.params(boundary=stk.CHANNEL, substeps_per_pass=1, brick=(4, 4, 4))
+    for sort_first in (True, False):
+        chunk, field = device(ctx, case, precision)
+        if sort_first:
+            stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
+        new = inputs.channel_injection(case, 1, 100)
+        new[1][17] = dt(case.ending[0])
+        n0 = chunk.count()
+        if sort_first:
+            with pytest.raises(_lib_error(), match="outside the mesh"):
+                chunk.inject(*new)
+            assert chunk.count() == n0
+        else:
+            chunk.inject(*new)
+            with pytest.raises(_lib_error(), match="outside the mesh"):
+                stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p, want_bbox=False)
+        chunk.close()
+        field.close()
+
+
+def _lib_error():
+    from scaletrack import _lib
+    return _lib.ScaleTrackError
