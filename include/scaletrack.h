@@ -135,10 +135,13 @@ int st_cloud_count(const st_cloud *cloud, int64_t *n);
  * `capacity` parcels.  st_cloud_inject appends k parcels (host SoA in the cloud's
  * precision; rho as in st_cloud_upload) after the current ones: they take the next
  * upload-order indices, so st_cloud_count grows by k and downloads return them last.
- * A sorted cloud stays sorted: the injected parcels are ordered by cell on the device
- * (queued on the context stream), and sorted steps advance them as an unsorted tail
- * (workgroups after the bricks in each pass's launch, one thread per parcel) until a
- * re-sort the device decides on (params->resort_fraction) takes them into the bricks.
+ * A sorted cloud stays sorted: the injected parcels are checked against its mesh at
+ * once (ST_E_RANGE, and nothing is appended, when a cell index
+ * floor((x - origin) / delta) lies outside [0, N); an unsorted cloud checks them at its
+ * next step), ordered by cell on the device (queued on the context stream), and sorted
+ * steps advance them as an unsorted tail (workgroups after the bricks in each pass's
+ * launch, one thread per parcel) until a re-sort the device decides on
+ * (params->resort_fraction) takes them into the bricks.
  * st_cloud_compact drops the parcels an open boundary removed (ST_CHANNEL; d < 0),
  * keeping the others in their upload order with their indices renumbered from 0, and
  * returns their number (may be NULL); the first sorted step after it sorts the cloud.
