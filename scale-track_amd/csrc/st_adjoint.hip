@@ -131,17 +131,32 @@ k_adjoint_segment(AdjArgs a) {
                 ++t;
             }
         }
-        // reverse this block; the U and G of each substep's cell (one 48-byte record)
-        // are requested one substep ahead, so the gather overlaps a substep of work
+        // reverse this block.  A parcel keeps its cell for many substeps, so each
+        // substep's U and G (one 48-byte record per cell) are read only when the cell
+        // differs from the later substep's.  The records of the block's last and first
+        // cells are requested together before the sweep; only a parcel that visited a
+        // third cell inside the block waits for a load in the sweep.
         double nvx = vx, nvy = vy, nvz = vz;  // velocity after the substep being reversed
-        double2 r0, r1, r2;
+        const uint32_t cl = rc[nb - 1], cf = rc[0];  // nb >= 1: every block holds a substep
+        double2 r0, r1, r2, f0, f1, f2;
+        load_ug(a.UG, cl, r0, r1, r2);
+        load_ug(a.UG, cf, f0, f1, f2);
+        uint32_t ccur = cl;
 #pragma unroll
         for (int j = B - 1; j >= 0; --j) {
             if (j < nb) {
-                if (j == nb - 1) load_ug(a.UG, rc[j], r0, r1, r2);
+                if (rc[j] != ccur) {
+                    ccur = rc[j];
+                    if (ccur == cf) {
+                        r0 = f0;
+                        r1 = f1;
+                        r2 = f2;
+                    } else {
+                        load_ug(a.UG, ccur, r0, r1, r2);
+                    }
+                }
                 const double uu[3] = {r0.x, r0.y, r1.x};
                 const double gg[3] = {r1.y, r2.x, r2.y};
-                if (j > 0) load_ug(a.UG, rc[j - 1], r0, r1, r2);
                 const unsigned bj = (rb >> (3 * j)) & 7u;
                 const double pv[3] = {rv[j][0], rv[j][1], rv[j][2]};
                 const double nv[3] = {nvx, nvy, nvz};
