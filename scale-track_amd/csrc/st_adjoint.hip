@@ -45,15 +45,22 @@ __device__ __forceinline__ void load_ug(const double *ug, uint32_t c, double2 &r
     r2 = q[2];
 }
 
-// Both instances are held to 168 VGPRs, 3 waves per SIMD; the kernel is bound by latency,
-// not issue.  The periodic one spills an output address (12 B, stored before the loops
-// and reloaded after them).  The wall one needs 188 VGPRs unconstrained and spills 100 B
-// at this cap; it measured 0.217 ms per 20-substep segment at 2^20 parcels against
-// 0.232 ms at 2 waves per SIMD without spills.
+// Both instances run at 3 waves per SIMD; the kernel is bound by latency, not issue.
+// The periodic one fits 166 VGPRs without spills when the compiler is allowed more (a
+// cap of 168 makes it spill 36 B); the wall one is held to 168 VGPRs and spills 84 B,
+// which measured faster than 2 waves per SIMD without spills.
+//
+// Workgroups are dispatched to the eight XCDs in turn, and each XCD has its own L2.
+// Workgroup w therefore takes parcel block (w mod 8) * (grid / 8) + w / 8, so each XCD
+// steps one contiguous eighth of the brick-sorted parcels: its gathers of U and G stay
+// within one region of the mesh, and its L2 holds that region.
 template <bool PERIODIC, int B>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERIODIC ? 2 : 3)))
 k_adjoint_segment(AdjArgs a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t w = blockIdx.x;
+    const uint32_t full = gridDim.x & ~7u;
+    if (w < full) w = (w & 7u) * (full >> 3) + (w >> 3);
+    const uint32_t p = w * blockDim.x + threadIdx.x;
     if (p >= a.np) return;
     const uint32_t n = a.np;
     const double dia = a.d[p];
@@ -85,20 +92,51 @@ k_adjoint_segment(AdjArgs a) {
         const int32_t part0 = cell.part;
         const double *q = a.U + 3u * cell.c;
         double ux = q[0], uy = q[1], uz = q[2];
-        int32_t t = 1;
-        const int32_t t0 = blk * B;
-        for (; t <= t0; ++t) {
-            double nx, ny, nz, nvx, nvy, nvz;
-            Cell nc;
-            advance<double, PERIODIC>(a.g, a.ph, f, qden, ux, uy, uz, px, py, pz, vx, vy, vz, cell, part0, nx,
-                                      ny, nz, nvx, nvy, nvz, nc);
-            if (t != a.K && nc.c != cell.c) {
-                cell = nc;
+        // Before each run of substeps (periodic form), the velocity of the cell the
+        // parcel would reach by drifting at its cell's velocity for the whole run is
+        // requested when that cell differs from the current one: a change into that cell
+        // then reads these registers instead of waiting for a load.  (In the wall form
+        // the extra registers cost more spills than the waits they save.)
+        uint32_t pc = 0xffffffffu;
+        double pu0 = 0.0, pu1 = 0.0, pu2 = 0.0;
+        auto prefetch_ahead = [&](int32_t steps) {
+            const double h = (double)steps * dt;
+            int32_t J[3];
+            J[0] = axis_floor(px + h * ux, a.g.o[0], a.g.rd[0]);
+            J[1] = axis_floor(py + h * uy, a.g.o[1], a.g.rd[1]);
+            J[2] = axis_floor(pz + h * uz, a.g.o[2], a.g.rd[2]);
+            Cell pcl;
+            cell_clamped(a.g, J, pcl);
+            if (pcl.c != cell.c) {
+                pc = pcl.c;
+                const double *qp = a.U + 3u * pc;
+                pu0 = qp[0];
+                pu1 = qp[1];
+                pu2 = qp[2];
+            }
+        };
+        auto enter = [&](const Cell &nc) {
+            cell = nc;
+            if (nc.c == pc) {
+                ux = pu0;
+                uy = pu1;
+                uz = pu2;
+            } else {
                 q = a.U + 3u * cell.c;
                 ux = q[0];
                 uy = q[1];
                 uz = q[2];
             }
+        };
+        int32_t t = 1;
+        const int32_t t0 = blk * B;
+        if (PERIODIC && t0 > 0) prefetch_ahead(t0);
+        for (; t <= t0; ++t) {
+            double nx, ny, nz, nvx, nvy, nvz;
+            Cell nc;
+            advance<double, PERIODIC>(a.g, a.ph, f, qden, ux, uy, uz, px, py, pz, vx, vy, vz, cell, part0, nx,
+                                      ny, nz, nvx, nvy, nvz, nc);
+            if (t != a.K && nc.c != cell.c) enter(nc);
             px = nx; py = ny; pz = nz;
             vx = nvx; vy = nvy; vz = nvz;
         }
@@ -107,6 +145,7 @@ k_adjoint_segment(AdjArgs a) {
         double rv[B][3];
         uint32_t rc[B];
         unsigned rb = 0;
+        if (PERIODIC) prefetch_ahead(nb);
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             if (j < nb) {
@@ -119,13 +158,7 @@ k_adjoint_segment(AdjArgs a) {
                 const unsigned bnc = advance<double, PERIODIC>(a.g, a.ph, f, qden, ux, uy, uz, px, py, pz, vx, vy,
                                                                vz, cell, part0, nx, ny, nz, nvx, nvy, nvz, nc);
                 rb |= bnc << (3 * j);
-                if (t != a.K && nc.c != cell.c) {
-                    cell = nc;
-                    q = a.U + 3u * cell.c;
-                    ux = q[0];
-                    uy = q[1];
-                    uz = q[2];
-                }
+                if (t != a.K && nc.c != cell.c) enter(nc);
                 px = nx; py = ny; pz = nz;
                 vx = nvx; vy = nvy; vz = nvz;
                 ++t;
