@@ -127,11 +127,18 @@ int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
     st_params p;
     ST_TRY(prepare(a->ctx, c, f, dt_fluid, a->nsub, &a->p, p));
     hipStream_t s = a->ctx->stream;
-    // sorted, and with no unsorted tail: the checkpoints follow the sorted passes
-    if (p.method == ST_METHOD_SORTED && (!sort_matches(c, f, p) || c->sorted_n != c->n)) ST_TRY(sort_any(c, f, p));
-    ST_TRY(st_field_zero_source(f));
+    // sorted, and with no unsorted tail: the checkpoints follow the sorted passes.  The
+    // adjoint's own hold on the cloud does not stop its forward from sorting the cloud
+    // for its parameters or growing the statistics slots: the forward defines what the
+    // adjoint holds.  Holds of graphs, batches and other adjoints still do.
+    unpin(c);
+    int r = ST_OK;
+    if (p.method == ST_METHOD_SORTED && (!sort_matches(c, f, p) || c->sorted_n != c->n)) r = sort_any(c, f, p);
     StatsBuf sb;
-    ST_TRY(cloud_stats(c, stats_slots(c, p), sb));
+    if (r == ST_OK) r = cloud_stats(c, stats_slots(c, p), sb);
+    pin(c);
+    ST_TRY(r);
+    ST_TRY(st_field_zero_source(f));
     a->dt_sub = substep_dt(ST_FP64, dt_fluid, a->nsub);
     const uint32_t n = a->n;
     for (int32_t seg = 0; seg < a->nseg; ++seg) {

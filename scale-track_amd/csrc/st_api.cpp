@@ -267,7 +267,8 @@ int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
     if (c->pins > 0 && c->nbricks > 0 && !same_geometry(c, f, p))
         return fail(ST_E_STATE,
                     "%d graph(s), batch(es) or adjoint(s) captured this cloud sorted by brick %dx%dx%d of its "
-                    "mesh; a sort by another brick or mesh would change what they step: destroy them first",
+                    "mesh; a sort by another brick, mesh or boundary form would change what they step: destroy "
+                    "them first",
                     c->pins, c->sort_bd[0], c->sort_bd[1], c->sort_bd[2]);
     const int32_t rows = sort_rows(c->n, nbr);
     ST_TRY(sort_buffers(c, nbr, true));

@@ -10,7 +10,7 @@ import pytest
 import scaletrack as stk
 from scaletrack.adjoint import Adjoint, source_misfit_rrule
 from oracle import st_oracle
-from scaletrack import inputs
+from scaletrack import _lib, inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -150,3 +150,41 @@ def test_adjoint_vs_frozen_event_fd(ctx, boundary):
     for which, adj_val in (("d", np.dot(gd, dd)), ("rho", np.dot(grho, dr)), ("cd", gcd)):
         fd = (L(+1, which) - L(-1, which)) / (2 * h)
         assert abs(fd - adj_val) <= 1e-4 * abs(fd), (which, fd, adj_val)
+
+
+def test_adjoint_forward_resorts_a_cloud_only_it_holds(ctx):
+    """A cloud stepped (and so sorted) for the wall form and another brick, then given to
+    an adjoint of the periodic form: the adjoint's forward re-sorts it for its own
+    parameters (its own hold on the cloud does not refuse that), and the loss and
+    gradients equal those of the same adjoint on a fresh upload of the same state.  A
+    graph holding the cloud still refuses the re-sort."""
+    case = make_case(np_parcels=2000, boundary=1, nsub=20)
+    T = target(case)
+    p = stk.params(boundary=1, brick=(4, 4, 4))
+    runs = []
+    for presorted in (False, True):
+        field, chunk = gpu_objects(ctx, case)
+        if presorted:
+            chunk.sort(ctx, field, stk.params(boundary=0, brick=(2, 3, 4)))
+        adj = Adjoint(ctx, chunk, field, case.nsub, 5, p)
+        adj.forward(case.dt_fluid)
+        loss = adj.source_misfit(T)
+        adj.backward()
+        runs.append((loss, adj.gradients()))
+        adj.close()
+        if presorted:
+            chunk.sort(ctx, field, stk.params(boundary=0, brick=(2, 3, 4)))
+            graph = stk.Graph(ctx, chunk, field, case.dt_fluid, case.nsub, stk.params(boundary=0, brick=(2, 3, 4)))
+            held = Adjoint(ctx, chunk, field, case.nsub, 5, p)
+            with pytest.raises(stk.ScaleTrackError) as ei:
+                held.forward(case.dt_fluid)
+            assert ei.value.status == _lib.ST_E_STATE and "boundary form" in str(ei.value)
+            held.close()
+            graph.close()
+        chunk.close()
+        field.close()
+    (l0, g0), (l1, g1) = runs
+    assert abs(l1 - l0) <= 1e-12 * abs(l0)
+    for k in ("d", "rho", "cd"):
+        a, b = np.asarray(g0[k], float), np.asarray(g1[k], float)
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max(), k
