@@ -19,9 +19,11 @@
  *
  * Lifetimes: a graph, batch or adjoint keeps raw device pointers of the clouds and
  * fields it was made from.  While one is alive those clouds and fields refuse (with
- * ST_E_STATE, before any device work) to be destroyed, to be sorted by another brick or
- * mesh, or to grow a buffer the graph or batch captured; destroy the graph, batch or
- * adjoint first.  A batch refuses to be destroyed while a graph of it is alive.
+ * ST_E_STATE, before any device work) to be destroyed, to be sorted by another brick,
+ * mesh or boundary form, or to grow a buffer the graph or batch captured; destroy the
+ * graph, batch or adjoint first.  An adjoint's own forward may still sort the cloud for
+ * the adjoint's parameters when no other graph, batch or adjoint holds it.  A batch
+ * refuses to be destroyed while a graph of it is alive.
  */
 #ifndef SCALETRACK_H
 #define SCALETRACK_H
