@@ -296,7 +296,7 @@ int sort_cloud(st_cloud *c, st_field *f, const st_params &p) {
 template <class T>
 static int queue_resort(st_cloud *c, st_field *f, const st_params &p, const unsigned int *gate) {
     const int64_t nbr = c->nbricks;
-    return enqueue_sort<T>(c, f, p, c->nb, nbr, sort_rows(c->n, nbr), gate, c->resort + 1);
+    return enqueue_sort<T>(c, f, p, c->nb, nbr, sort_rows(c->n, nbr), gate, c->resort + 2);
 }
 
 bool sort_matches(const st_cloud *c, const st_field *f, const st_params &p) {
@@ -386,8 +386,8 @@ int32_t stats_slots(const st_cloud *c, const st_params &p) {
 static void observe_step(st_cloud *c) {
     if (!c->strays_pending || hipEventQuery(c->strays_ev) != hipSuccess) return;
     c->strays_pending = false;
-    c->last_stray_fraction = c->strays_den > 0 ? (double)c->strays_host[0] / (double)c->strays_den : 0.0;
-    if (c->strays_host[1] && c->sorted) c->sorted_n = std::max(c->sorted_n, c->strays_den);
+    c->last_stray_fraction = c->strays_den > 0 ? (double)c->strays_host[1] / (double)c->strays_den : 0.0;
+    if (c->strays_host[0] && c->sorted) c->sorted_n = std::max(c->sorted_n, c->strays_den);
 }
 
 int cloud_stats(st_cloud *c, int32_t nslots, StatsBuf &sb) {
@@ -395,7 +395,6 @@ int cloud_stats(st_cloud *c, int32_t nslots, StatsBuf &sb) {
     sb.part = c->part;
     sb.cap = c->part_cap;
     sb.bbox = c->bbox;
-    sb.strays = c->strays;
     sb.gate = c->resort;
     return ST_OK;
 }
@@ -474,21 +473,21 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
             resort = false;  // no room to re-sort without allocating: the tail stays
         }
     }
-    StatsTarget t{sb.part, nslots, sb.bbox, sb.strays, nullptr, 0u};
+    // The gate is written every step: 0 unless a re-sort was queued and is due.
+    StatsTarget t{sb.part, nslots, sb.bbox, sb.gate + 1, sb.gate, 0xffffffffu};
     if (resort) {
         // share > resort_fraction  <=>  strays > floor(resort_fraction * n)
         const double th = std::floor(p.resort_fraction * (double)c->n);
-        t.resort = sb.gate;
         t.threshold = th >= 4294967295.0 ? 0xffffffffu : (uint32_t)th;
     }
     ST_HIP(launch_reduce_stats(t, s));
     if (p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER) f->stray_dirty = true;
     if (resort) ST_TRY(queue_resort<T>(c, f, p, sb.gate));
     if (p.method == ST_METHOD_SORTED && !capturing) {
-        ST_HIP(hipMemcpyAsync(c->strays_host, sb.strays, 4, hipMemcpyDeviceToHost, s));
-        // the gate this step's re-sort ran on, or a device word that is always 0 (the
-        // host never writes the pinned words: an earlier copy may still be in flight)
-        ST_HIP(hipMemcpyAsync(c->strays_host + 1, resort ? sb.gate : c->resort + 2, 4, hipMemcpyDeviceToHost, s));
+        // the gate this step's re-sort ran on (0 when none was queued) and the strays,
+        // in one copy (the host never writes the pinned words: an earlier copy may still
+        // be in flight)
+        ST_HIP(hipMemcpyAsync(c->strays_host, sb.gate, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
         ST_HIP(hipEventRecord(c->strays_ev, s));
         c->strays_pending = true;
         c->strays_den = c->n;
@@ -708,7 +707,6 @@ int st_cloud_create(st_ctx *ctx, int32_t precision, int64_t n, double mu_c, doub
     int r = alloc_set(c, 0);
     hipError_t e = hipSuccess;
     if (r == ST_OK) e = hipMalloc((void **)&c->bbox, 6 * sizeof(double));
-    if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->strays, 4);
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->bad, 4);
     if (r == ST_OK && e == hipSuccess) e = hipMalloc((void **)&c->resort, 4 * sizeof(unsigned int));
     if (r == ST_OK && e == hipSuccess) e = st::launch_fill_u32(c->resort, 0u, 4, ctx->stream);
@@ -745,7 +743,6 @@ int st_cloud_destroy(st_cloud *c) {
     if (c->stage_ev) hipEventDestroy(c->stage_ev);
     if (c->bbox) hipFree(c->bbox);
     if (c->part) hipFree(c->part);
-    if (c->strays) hipFree(c->strays);
     if (c->bad) hipFree(c->bad);
     if (c->resort) hipFree(c->resort);
     if (c->strays_host) hipHostFree(c->strays_host);
@@ -1156,7 +1153,7 @@ int st_cloud_resort_count(st_cloud *c, int64_t *count) {
     if (!c || !count) return fail(ST_E_INVALID, "null argument");
     ST_TRY(check_ctx(c->ctx));
     unsigned int v = 0;
-    ST_HIP(hipMemcpyAsync(&v, c->resort + 1, sizeof v, hipMemcpyDeviceToHost, c->ctx->stream));
+    ST_HIP(hipMemcpyAsync(&v, c->resort + 2, sizeof v, hipMemcpyDeviceToHost, c->ctx->stream));
     ST_HIP(hipStreamSynchronize(c->ctx->stream));
     *count = (int64_t)v;
     return ST_OK;

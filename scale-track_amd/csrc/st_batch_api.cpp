@@ -102,7 +102,7 @@ int build_table(st_batch *b, double dt_sub) {
         ST_TRY(ensure_part(c, c->nbricks));
         a.part = c->part;
         a.nbricks = c->nbricks;
-        targets[i] = st::StatsTarget{c->part, c->nbricks, c->bbox, c->strays};
+        targets[i] = st::StatsTarget{c->part, c->nbricks, c->bbox, c->resort + 1};
         for (int k = 0; k < 3; ++k) {
             a.nb[k] = c->nb[k];
             a.bd[k] = p.brick[k];

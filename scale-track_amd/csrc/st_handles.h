@@ -91,9 +91,8 @@ struct st_cloud {
     double *bbox = nullptr;              // device, 6: min xyz, max xyz of the last call
     double *part = nullptr;              // device, per-workgroup statistics slots
     int32_t part_cap = 0;
-    unsigned int *strays = nullptr;      // device
     unsigned int *bad = nullptr;         // device
-    unsigned int *strays_host = nullptr; // pinned: [0] strays of the last step, [1] its re-sort gate
+    unsigned int *strays_host = nullptr; // pinned: [0] the last step's re-sort gate, [1] its strays
     hipEvent_t strays_ev = nullptr;
     bool strays_pending = false;
     int64_t strays_den = 0;  // parcels behind strays_host
@@ -105,8 +104,9 @@ struct st_cloud {
     // most).
     int64_t sorted_n = 0;
     double last_stray_fraction = -1.0;
-    // device: [0] gate of the re-sort queued behind the last step (set by the statistics
-    // reduction), [1] re-sorts the device decided on so far, [2] always 0
+    // device: [0] gate of the re-sort queued behind the last step and [1] that step's
+    // stray count (both written by the statistics reduction, read back as one pair),
+    // [2] re-sorts the device decided on so far, [3] unused
     unsigned int *resort = nullptr;
     // Graphs, batches and adjoints holding raw device pointers of this cloud: while any
     // is alive the cloud refuses to be destroyed, re-sorted with another brick geometry,
@@ -177,8 +177,7 @@ struct StatsBuf {
     double *part = nullptr;
     int32_t cap = 0;
     double *bbox = nullptr;
-    unsigned int *strays = nullptr;
-    unsigned int *gate = nullptr;
+    unsigned int *gate = nullptr;    // gate[0]: re-sort gate, gate[1]: stray count
     bool may_alloc = true;  // the device re-sort may size the cloud's sort buffers first
 };
 // The cloud's own statistics buffers (slots grown to nslots on demand).

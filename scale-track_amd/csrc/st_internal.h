@@ -51,7 +51,8 @@ constexpr int kStatSlot = 8;  // doubles per slot
 // Where one cloud's per-slot statistics are reduced to.  With `resort` set, the
 // reduction also writes resort[0] = 1 when more than `threshold` parcels were outside
 // their brick (else 0): the gate of the re-sort queued behind the pass, so the
-// decision is taken on the device without a host round trip.
+// decision is taken on the device without a host round trip (a threshold of
+// 0xffffffff writes 0: no re-sort).
 struct StatsTarget {
     const double *part;  // nslots * kStatSlot
     int32_t nslots;

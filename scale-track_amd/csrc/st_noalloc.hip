@@ -22,7 +22,7 @@
 namespace {
 
 constexpr int kMaxDevices = 64;
-constexpr size_t kHeaderBytes = 256;  // bbox (6 doubles), stray count, re-sort gate
+constexpr size_t kHeaderBytes = 256;  // bbox (6 doubles), re-sort gate, stray count
 constexpr int32_t kMaxSlots = 65536;  // statistics slots: bricks (sorted) or workgroups (plain)
 constexpr size_t kScratchBytes = kHeaderBytes + (size_t)kMaxSlots * st::kStatSlot * sizeof(double);  // 4 MiB + 256 B
 
@@ -87,8 +87,7 @@ extern "C" int st_evolve_noalloc(st_ctx *ctx, st_cloud *c, st_field *f, double d
                     nslots, p.method == ST_METHOD_SORTED ? "one per brick" : "one per workgroup", kMaxSlots);
     StatsBuf sb;
     sb.bbox = (double *)base;
-    sb.strays = (unsigned int *)(base + 6 * sizeof(double));
-    sb.gate = sb.strays + 1;
+    sb.gate = (unsigned int *)(base + 6 * sizeof(double));  // [0] re-sort gate, [1] stray count
     sb.part = (double *)(base + kHeaderBytes);
     sb.cap = kMaxSlots;
     sb.may_alloc = false;  // a re-sort runs only if the cloud's sort buffers already fit

@@ -919,17 +919,37 @@ template hipError_t launch_pass_sorted<double>(const PassArgs<double> &, int, in
 // ---------------------------------------------------------------------------------
 // Small utility kernels (graph-capturable: no hipMemsetAsync anywhere).
 
-// Reduces one cloud's statistics slots: one workgroup of 256 threads per target.
+// Reduces one cloud's statistics slots: one workgroup of kReduceBlock threads per
+// target.  Each thread requests kReducePer slots before it combines any of them, so a
+// cloud of up to kReduceBlock * kReducePer slots (4096 bricks and more) costs one memory
+// round trip instead of one per slot a thread visits.  min, max and the sum of integer
+// counts are exact in any order.
+constexpr int kReduceBlock = 1024;
+constexpr int kReducePer = 4;
+
 __device__ __forceinline__ void reduce_stats(const StatsTarget &t) {
-    __shared__ double red[7][256 / kWave];
+    __shared__ double red[7][kReduceBlock / kWave];
     double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
-    for (int32_t k = threadIdx.x; k < t.nslots; k += blockDim.x) {
-        const double *q = t.part + (size_t)k * kStatSlot;
-        for (int c = 0; c < 3; ++c) {
-            v[c] = fmin(v[c], q[c]);
-            v[3 + c] = fmax(v[3 + c], q[3 + c]);
+    for (int32_t k0 = threadIdx.x; k0 < t.nslots; k0 += kReducePer * kReduceBlock) {
+        double q[kReducePer][7];
+        bool ok[kReducePer];
+#pragma unroll
+        for (int r = 0; r < kReducePer; ++r) {
+            const int32_t k = k0 + r * kReduceBlock;
+            ok[r] = k < t.nslots;
+            const double *slot = t.part + (size_t)(ok[r] ? k : k0) * kStatSlot;  // k0 < nslots
+#pragma unroll
+            for (int c = 0; c < 7; ++c) q[r][c] = slot[c];
         }
-        v[6] += q[6];
+#pragma unroll
+        for (int r = 0; r < kReducePer; ++r) {
+            if (!ok[r]) continue;
+            for (int c = 0; c < 3; ++c) {
+                v[c] = fmin(v[c], q[r][c]);
+                v[3 + c] = fmax(v[3 + c], q[r][3 + c]);
+            }
+            v[6] += q[r][6];
+        }
     }
     for (int off = kWave / 2; off > 0; off >>= 1) {
         for (int c = 0; c < 3; ++c) {
@@ -943,7 +963,7 @@ __device__ __forceinline__ void reduce_stats(const StatsTarget &t) {
         for (int c = 0; c < 7; ++c) red[c][wid] = v[c];
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) {
+        for (int w = 1; w < kReduceBlock / kWave; ++w) {
             for (int c = 0; c < 3; ++c) {
                 v[c] = fmin(v[c], red[c][w]);
                 v[3 + c] = fmax(v[3 + c], red[3 + c][w]);
@@ -957,17 +977,19 @@ __device__ __forceinline__ void reduce_stats(const StatsTarget &t) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_reduce_stats(StatsTarget t) { reduce_stats(t); }
-__global__ void __launch_bounds__(256) k_reduce_stats_many(const StatsTarget *ts) { reduce_stats(ts[blockIdx.x]); }
+__global__ void __launch_bounds__(kReduceBlock) k_reduce_stats(StatsTarget t) { reduce_stats(t); }
+__global__ void __launch_bounds__(kReduceBlock) k_reduce_stats_many(const StatsTarget *ts) {
+    reduce_stats(ts[blockIdx.x]);
+}
 
 hipError_t launch_reduce_stats(const StatsTarget &t, hipStream_t s) {
-    k_reduce_stats<<<1, 256, 0, s>>>(t);
+    k_reduce_stats<<<1, kReduceBlock, 0, s>>>(t);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_stats_many(const StatsTarget *targets, int32_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    k_reduce_stats_many<<<n, 256, 0, s>>>(targets);
+    k_reduce_stats_many<<<n, kReduceBlock, 0, s>>>(targets);
     return hipGetLastError();
 }
 
