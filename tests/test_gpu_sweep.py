@@ -163,3 +163,106 @@ def test_random_batch(ctx, seed):
     for ch, f in objs:
         ch.close()
         f.close()
+
+
+def draw_channel(seed):
+    """A random channel-form configuration (walls on y and z, open x planes): mesh,
+    carrier with a mean flow along x that carries parcels out through the outlet, cloud
+    (optionally with a per-parcel density), substeps, method, brick, block, flush and
+    re-sort share; and the parcels injected before each call (0 to 3000, spread over
+    the first fifth of the duct) and the calls after which removed parcels are dropped."""
+    rng = np.random.default_rng(9000 + seed)
+    precision = stk.FP32 if rng.random() < 0.5 else stk.FP64
+    dt = np.float32 if precision == stk.FP32 else np.float64
+    n = (int(rng.integers(2, 41)), int(rng.integers(1, 21)), int(rng.integers(1, 21)))
+    origin = tuple(float(v) for v in rng.uniform(-1.0, 1.0, size=3))
+    ending = tuple(o + float(L) for o, L in zip(origin, rng.uniform(0.05, 2.0, size=3)))
+    amp = float(10 ** rng.uniform(-1, 0.5))
+    U = inputs.fourier_field(n, origin, ending, modes=5, seed=seed, amplitude=0.3 * amp)
+    U[:, 0] += amp
+    U = U.astype(dt)
+    npc = int(rng.choice([0, 1, 64, int(rng.integers(100, 15001)), int(rng.integers(100, 15001))]))
+    x, y, z = inputs.positions(seed, npc, origin, ending, dtype=dt)
+    d = inputs.lognormal_diameters(seed, npc, median=float(10 ** rng.uniform(-5, -3.8)), dtype=dt)
+    u, v, w = [(rng.standard_normal(npc) * 0.3 * amp).astype(dt) for _ in range(3)]
+    rho = (1500.0 * rng.uniform(0.5, 2.0, size=npc)).astype(dt) if rng.random() < 0.3 else None
+    L0 = ending[0] - origin[0]
+    nsub = int(rng.integers(1, 13))
+    # a fluid step moves the mean flow over about 0.3 to 3 duct lengths' worth of cells
+    dt_fluid = float(L0 / amp * 10 ** rng.uniform(-1.5, -0.5))
+    case = inputs.Case(n, origin, ending, U, d, x, y, z, u, v, w, rho=rho, mu_c=1.8e-5, rho_d=1500.0, rho_c=1.2,
+                       dt_fluid=dt_fluid, nsub=nsub, boundary=2, name=f"channel sweep {seed}")
+    elem = 8 if precision == stk.FP64 else 4
+    while True:
+        brick = tuple(int(v) for v in rng.integers(1, 17, size=3))
+        tile = (brick[0] + 2) * (brick[1] + 2) * (brick[2] + 2)
+        if brick[0] * brick[1] * brick[2] <= 2048 and tile * 3 * (8 + elem) <= LDS_LIMIT:
+            break
+    p = stk.params(boundary=stk.CHANNEL, method=stk.SORTED if rng.random() < 0.7 else stk.PLAIN,
+                   substeps_per_pass=int(rng.choice([0, 1, 2, nsub])), brick=brick,
+                   block=int(rng.choice([64, 128, 256, 512, 1024])),
+                   flush=stk.FLUSH_OWNER if rng.random() < 0.7 else stk.FLUSH_ATOMIC,
+                   resort_fraction=float(rng.choice([0.0, 0.02, 0.1, 1.0])))
+    injections = []
+    for call in range(4):
+        k = int(rng.choice([0, 1, 50, int(rng.integers(100, 3001))]))
+        xi, yi, zi = inputs.positions(100 * seed + call, k, origin, ending, dtype=dt)
+        xi = (origin[0] + (xi.astype(np.float64) - origin[0]) * 0.2).astype(dt)
+        lo = dt(origin[0])
+        if float(lo) < origin[0]:  # the nearest value of the state type at or above the inlet
+            lo = np.nextafter(lo, dt(np.inf))
+        xi = np.maximum(xi, lo)
+        di = inputs.lognormal_diameters(100 * seed + call, k, median=float(10 ** rng.uniform(-5, -3.8)), dtype=dt)
+        ui = np.full(k, amp, dtype=dt)
+        vi, wi = np.zeros(k, dtype=dt), np.zeros(k, dtype=dt)
+        rhoi = (1500.0 * rng.uniform(0.5, 2.0, size=k)).astype(dt) if rho is not None else None
+        injections.append(((di, xi, yi, zi, ui, vi, wi), rhoi, bool(rng.random() < 0.4)))
+    return case, precision, p, injections
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_channel(ctx, seed):
+    """Random channel runs: four calls, parcels injected before each and the removed ones
+    dropped after some; the cloud in upload order, removal marks included, equals the
+    oracle's list kept the same way (appended, then filtered in order)."""
+    case, precision, p, injections = draw_channel(seed)
+    ident = (f"channel seed {seed}: n={case.n} np={case.np} {'fp32' if precision == stk.FP32 else 'fp64'} "
+             f"method={p.method} brick={tuple(p.brick)} block={p.block} flush={p.flush} nsub={case.nsub} "
+             f"per={p.substeps_per_pass} resort={p.resort_fraction} "
+             f"rho={'parcel' if case.rho is not None else 'chunk'}")
+    field = stk.TwoWayEulerian(ctx, stk.Mesh(case.n, case.origin, case.ending), precision)
+    field.upload_U(case.U)
+    chunk = stk.Chunk(ctx, case.np, case.mu_c, case.rho_d, case.rho_d_by_rho_c, precision, rho_p=case.rho,
+                      rho_c=case.rho_c)
+    chunk.set_state(*case.state())
+    chunk.upload()
+    st = case.state()
+    rho = case.rho
+    oprec = st_oracle.FP32 if precision == stk.FP32 else st_oracle.FP64
+    for call, (new, rho_new, drop) in enumerate(injections):
+        if new[0].size:
+            chunk.inject(*new, rho=rho_new)
+            st = [np.concatenate([a, b]) for a, b in zip(st, new)]
+            if rho is not None:
+                rho = np.concatenate([rho, rho_new])
+        field.zero_source()
+        bb = stk.evolve_device(ctx, chunk, field, case.dt_fluid, case.nsub, p)
+        src = field.download_source()
+        ref_src, ref_bb = st_oracle.evolve(st, case.U, case.n, case.origin, case.ending, case.dt_fluid,
+                                           nsub=case.nsub, precision=oprec, boundary=st_oracle.CHANNEL,
+                                           mu_c=case.mu_c, rho_d=case.rho_d, rho_d_by_rho_c=case.rho_d_by_rho_c,
+                                           rho_c=case.rho_c, rho=rho, substeps_per_pass=p.substeps_per_pass)
+        scale = max(np.abs(ref_src).max(), 1e-300)
+        assert np.abs(src - ref_src).max() <= 1e-12 * scale, f"{ident}, call {call}: source"
+        np.testing.assert_array_equal(np.concatenate(bb), ref_bb, err_msg=f"{ident}, call {call}: bbox")
+        for name, a, b in zip("dxyzuvw", chunk.download(), st):
+            bad = np.flatnonzero(a != b)
+            assert bad.size == 0, f"{ident}, call {call}: {name} differs for {bad.size} parcels, first {bad[:3]}"
+        if drop:
+            live = st[0] > 0
+            st = [a[live] for a in st]
+            if rho is not None:
+                rho = rho[live]
+            assert chunk.compact() == st[0].size, f"{ident}, call {call}: live count"
+    chunk.close()
+    field.close()
