@@ -920,10 +920,10 @@ template hipError_t launch_pass_sorted<double>(const PassArgs<double> &, int, in
 // Small utility kernels (graph-capturable: no hipMemsetAsync anywhere).
 
 // Reduces one cloud's statistics slots: one workgroup of kReduceBlock threads per
-// target.  Each thread requests kReducePer slots before it combines any of them, so a
-// cloud of up to kReduceBlock * kReducePer slots (4096 bricks and more) costs one memory
-// round trip instead of one per slot a thread visits.  min, max and the sum of integer
-// counts are exact in any order.
+// target.  Each thread requests kReducePer slots before it combines any of them, so
+// every kReduceBlock * kReducePer slots (4096) cost one memory round trip instead of one
+// per slot a thread visits.  min, max and the sum of integer counts are exact in any
+// order.
 constexpr int kReduceBlock = 1024;
 constexpr int kReducePer = 4;
 
