@@ -180,10 +180,9 @@ template <class T, int BND, bool RHO, class Env, class Prefetch>
 __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
                                             const ParcelIn<T> &in, uint32_t p, int32_t nsteps,
                                             Env &env, Box<T> &box, const Prefetch &prefetch) {
-    if (BND == kChannel && !(in.d > (T)0)) {  // removed earlier: never touched again
-        prefetch();
-        return;
-    }
+    // removed earlier (channel form): no substep, nothing stored, deposited or counted,
+    // and the next parcel's loads are issued from the same place as for the others
+    const bool live = BND != kChannel || in.d > (T)0;
     double m;
     T f, den;
     parcel_consts(ph, RHO, in.d, in.rho, m, f, den);
@@ -196,15 +195,16 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
     Cell cell;
     locate<T, BND == kPeriodic>(g, px, py, pz, I, cell);  // :1286
     const int32_t part0 = cell.part;
-    env.enter(cell);
+    if (live) env.enter(cell);
     typename Env::Slot sl = env.slot(cell);
-    T ux, uy, uz;
-    env.load_u(cell, sl, ux, uy, uz);  // :1287-1288
+    T ux = (T)0, uy = (T)0, uz = (T)0;
+    if (live) env.load_u(cell, sl, ux, uy, uz);  // :1287-1288
     double s0x = (double)vx, s0y = (double)vy, s0z = (double)vz;  // :1289
     prefetch();
 
     bool exited = false;
-    for (int32_t t = 1; t <= nsteps; ++t) {
+    const int32_t steps = live ? nsteps : 0;
+    for (int32_t t = 1; t <= steps; ++t) {
         T nx, ny, nz, nvx, nvy, nvz;
         Cell nc;
         const unsigned b = advance<T, BND>(g, ph, f, q, ux, uy, uz, px, py, pz, vx, vy, vz, cell,
@@ -239,6 +239,7 @@ __device__ __forceinline__ void step_parcel(const Geo &g, const Phys<T> &ph, con
         pz = nz;
     }
     // :1355-1361
+    if (!live) return;
     const uint32_t off = p * (uint32_t)sizeof(T);
     st_off(s.x, off, px);
     st_off(s.y, off, py);
@@ -262,10 +263,9 @@ template <class T, int BND, bool RHO, class Env, class Prefetch>
 __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph, const Soa<T> &s,
                                                 const ParcelIn<T> &in, uint32_t p, Env &env, Box<T> &box,
                                                 bool stats, const Prefetch &prefetch) {
-    if (BND == kChannel && !(in.d > (T)0)) {  // removed earlier: never touched again
-        prefetch();
-        return;
-    }
+    // removed earlier (channel form): stepped like the others but nothing is stored,
+    // deposited or counted, so the next parcel's loads are issued from one place
+    const bool live = BND != kChannel || in.d > (T)0;
     double m;
     T f, den;
     parcel_consts(ph, RHO, in.d, in.rho, m, f, den);
@@ -275,10 +275,10 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
     int32_t I[3];
     Cell cell;
     locate<T, BND == kPeriodic>(g, px, py, pz, I, cell);  // :1286
-    env.enter(cell);
+    if (live) env.enter(cell);
     const typename Env::Slot sl = env.slot(cell);
-    T ux, uy, uz;
-    env.load_u(cell, sl, ux, uy, uz);  // :1287-1288
+    T ux = (T)0, uy = (T)0, uz = (T)0;
+    if (live) env.load_u(cell, sl, ux, uy, uz);  // :1287-1288
     prefetch();
 
     // :1295-1296
@@ -325,6 +325,7 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
         else if (s2 > 0) nz = nz - ph.L[2];
     }
     // :1355-1361
+    if (!live) return;
     const uint32_t off = p * (uint32_t)sizeof(T);
     st_off(s.x, off, nx);
     st_off(s.y, off, ny);
