@@ -732,8 +732,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeo
 
 // A pass over a sorted cloud with an unsorted tail (parcels injected since its last
 // sort) in one launch: workgroups 0 .. nbricks - 1 step the bricks, the ones after them
-// the tail, one thread per parcel from a.tail0[0] on.  The tail's workgroups come last
-// in dispatch order, so they fill the slots the bricks leave while the launch drains.
+// the tail, one thread per parcel from a.tail0[0] on.  With G workgroups of which
+// n = G - nbricks are tail workgroups, workgroup w runs tail workgroup floor(w n / G)
+// when floor((w + 1) n / G) exceeds it, else brick w - floor(w n / G): the tail's
+// latency-bound workgroups are spread evenly through dispatch order and overlap the
+// bricks' bandwidth-bound sweep (tail workgroups last, or first, are slower: the drain
+// lengthens, or the sweep starts late).
 // Tail deposits go where deposits from outside a brick go -- `stray` with atomics under
 // the owner flush (bricks update their own cells of `src` with plain read-modify-writes
 // in the same launch), `src` with atomics under the atomic flush.  Statistics slots:
@@ -742,12 +746,13 @@ template <class T, int BND, bool RHO>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4)))
 k_pass_sorted_tail(PassArgs<T> a) {
     extern __shared__ double lds[];
-    if ((int32_t)blockIdx.x < a.nbricks) {
-        process_brick<T, BND, RHO, false>(a, blockIdx.x, a.nsteps, lds, a.part);
+    const uint64_t G = gridDim.x, n = G - (uint64_t)a.nbricks, w = blockIdx.x;
+    const uint32_t k0 = (uint32_t)(w * n / G), k1 = (uint32_t)((w + 1) * n / G);
+    if (k1 > k0) {
+        plain_block<T, BND, RHO, true>(a, (uint32_t)a.tail0[0], k0, a.flush == 0 ? a.stray : a.src,
+                                       a.part ? a.part + (size_t)(a.nbricks + k0) * kStatSlot : nullptr);
     } else {
-        plain_block<T, BND, RHO, true>(a, (uint32_t)a.tail0[0], blockIdx.x - (uint32_t)a.nbricks,
-                                       a.flush == 0 ? a.stray : a.src,
-                                 a.part ? a.part + (size_t)blockIdx.x * kStatSlot : nullptr);
+        process_brick<T, BND, RHO, false>(a, (int32_t)(w - k0), a.nsteps, lds, a.part);
     }
 }
 
