@@ -91,8 +91,10 @@ struct st_cloud {
     double *bbox = nullptr;              // device, 6: min xyz, max xyz of the last call
     double *part = nullptr;              // device, per-workgroup statistics slots
     int32_t part_cap = 0;
-    unsigned int *bad = nullptr;         // device
-    unsigned int *strays_host = nullptr; // pinned: [0] the last step's re-sort gate, [1] its strays
+    unsigned int *bad = nullptr;         // device: parcels outside the mesh (the range check's count)
+    unsigned int *strays_host = nullptr; // pinned: [0] the last step's re-sort gate, [1] its strays,
+                                         // written by the statistics reduction
+    unsigned int *strays_host_dev = nullptr;  // the same words' device address
     hipEvent_t strays_ev = nullptr;
     bool strays_pending = false;
     int64_t strays_den = 0;  // parcels behind strays_host

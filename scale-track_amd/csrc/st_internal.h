@@ -60,6 +60,7 @@ struct StatsTarget {
     unsigned int *strays;
     unsigned int *resort;  // or nullptr
     uint32_t threshold;
+    unsigned int *host;    // pinned host words [0] resort[0], [1] strays[0], or nullptr
 };
 
 template <class T>
@@ -143,14 +144,14 @@ hipError_t launch_reduce_stats_many(const StatsTarget *targets, int32_t n, hipSt
 template <class T>
 hipError_t launch_brick_keys(const Soa<T> &s, uint32_t np, Geo g, const int32_t nb[3],
                              const int32_t bd[3], int periodic, uint32_t *keys,
-                             uint32_t *counts, unsigned int *bad, const unsigned int *gate, hipStream_t st);
+                             uint32_t *counts, const unsigned int *gate, hipStream_t st);
 // Counting sort by brick in per-workgroup chunks (nbins * 4 B of LDS per workgroup):
 // keys, per-chunk histograms (nrows x nbins in hist), brick starts and the scatter.
 template <class T>
 hipError_t launch_sort_chunked(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out, uint32_t np,
                                Geo g, const int32_t nb[3], const int32_t bd[3], int periodic, int32_t nrows,
                                uint32_t *keys, uint32_t *hist, uint32_t *counts, int32_t *start,
-                               uint32_t *cursor, unsigned int *bad, const unsigned int *gate, hipStream_t s);
+                               uint32_t *cursor, const unsigned int *gate, hipStream_t s);
 hipError_t launch_scan(const uint32_t *counts, int32_t nbins, int32_t *start, uint32_t *cursor,
                        const unsigned int *gate, hipStream_t s);
 template <class T>

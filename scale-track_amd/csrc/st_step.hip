@@ -978,8 +978,13 @@ __device__ __forceinline__ void reduce_stats(const StatsTarget &t) {
         }
         for (int c = 0; c < 6; ++c) t.bbox[c] = v[c];
         const unsigned int strays = (unsigned int)v[6];
+        const unsigned int gate = t.resort && strays > t.threshold ? 1u : 0u;
         t.strays[0] = strays;
-        if (t.resort) t.resort[0] = strays > t.threshold ? 1u : 0u;
+        if (t.resort) t.resort[0] = gate;
+        if (t.host) {  // straight to the host: no copy launch behind the step
+            t.host[0] = gate;
+            t.host[1] = strays;
+        }
     }
 }
 
@@ -1117,7 +1122,7 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t *ctr, uint32_t key, bool
 template <class T, bool PERIODIC>
 __global__ void k_brick_keys(Soa<T> s, uint32_t np, Geo g, int32_t nb0, int32_t nb1, int32_t bd0,
                              int32_t bd1, int32_t bd2, uint32_t *keys, uint32_t *counts,
-                             unsigned int *bad, const unsigned int *gate) {
+                             const unsigned int *gate) {
     if (gated_off(gate)) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = p < np;
@@ -1126,7 +1131,6 @@ __global__ void k_brick_keys(Soa<T> s, uint32_t np, Geo g, int32_t nb0, int32_t 
         int32_t I[3];
         Cell cl;
         locate<T, PERIODIC>(g, s.x[p], s.y[p], s.z[p], I, cl);
-        if (!PERIODIC && cl.part != 0) atomicAdd(bad, 1u);
         key = (uint32_t)(cl.f[0] / bd0) +
               (uint32_t)nb0 * ((uint32_t)(cl.f[1] / bd1) + (uint32_t)nb1 * (uint32_t)(cl.f[2] / bd2));
         keys[p] = key;
@@ -1137,23 +1141,23 @@ __global__ void k_brick_keys(Soa<T> s, uint32_t np, Geo g, int32_t nb0, int32_t 
 template <class T>
 hipError_t launch_brick_keys(const Soa<T> &s, uint32_t np, Geo g, const int32_t nb[3],
                              const int32_t bd[3], int periodic, uint32_t *keys, uint32_t *counts,
-                             unsigned int *bad, const unsigned int *gate, hipStream_t st) {
+                             const unsigned int *gate, hipStream_t st) {
     if (!np) return hipSuccess;
     const uint32_t grid = (np + 255) / 256;
     if (periodic)
         k_brick_keys<T, true><<<grid, 256, 0, st>>>(s, np, g, nb[0], nb[1], bd[0], bd[1], bd[2], keys,
-                                                     counts, bad, gate);
+                                                     counts, gate);
     else
         k_brick_keys<T, false><<<grid, 256, 0, st>>>(s, np, g, nb[0], nb[1], bd[0], bd[1], bd[2], keys,
-                                                      counts, bad, gate);
+                                                      counts, gate);
     return hipGetLastError();
 }
 template hipError_t launch_brick_keys<float>(const Soa<float> &, uint32_t, Geo, const int32_t *,
                                              const int32_t *, int, uint32_t *, uint32_t *,
-                                             unsigned int *, const unsigned int *, hipStream_t);
+                                             const unsigned int *, hipStream_t);
 template hipError_t launch_brick_keys<double>(const Soa<double> &, uint32_t, Geo, const int32_t *,
                                               const int32_t *, int, uint32_t *, uint32_t *,
-                                              unsigned int *, const unsigned int *, hipStream_t);
+                                              const unsigned int *, hipStream_t);
 
 // Exclusive scan of the bin counts in one workgroup of 1024 threads: each thread
 // sums a contiguous segment, the 1024 partial sums are scanned in LDS, then each
@@ -1202,25 +1206,21 @@ template <class T, bool PERIODIC>
 __global__ void __launch_bounds__(kSortBlock) k_sort_count(Soa<T> s, uint32_t np, uint32_t chunk, Geo g,
                                                            int32_t nb0, int32_t nb1, int32_t bd0, int32_t bd1,
                                                            int32_t bd2, int32_t nbins, uint32_t *keys,
-                                                           uint32_t *hist, unsigned int *bad,
-                                                           const unsigned int *gate) {
+                                                           uint32_t *hist, const unsigned int *gate) {
     if (gated_off(gate)) return;
     extern __shared__ uint32_t h[];
     for (int32_t b = threadIdx.x; b < nbins; b += kSortBlock) h[b] = 0u;
     __syncthreads();
     const uint32_t p0 = blockIdx.x * chunk, p1 = min(p0 + chunk, np);
-    uint32_t nbad = 0;
     for (uint32_t p = p0 + threadIdx.x; p < p1; p += kSortBlock) {
         int32_t I[3];
         Cell cl;
         locate<T, PERIODIC>(g, s.x[p], s.y[p], s.z[p], I, cl);
-        if (!PERIODIC && cl.part != 0) ++nbad;
         const uint32_t key = (uint32_t)(cl.f[0] / bd0) +
                              (uint32_t)nb0 * ((uint32_t)(cl.f[1] / bd1) + (uint32_t)nb1 * (uint32_t)(cl.f[2] / bd2));
         keys[p] = key;
         atomicAdd(&h[key], 1u);
     }
-    if (nbad) atomicAdd(bad, nbad);
     __syncthreads();
     uint32_t *row = hist + (size_t)blockIdx.x * nbins;
     for (int32_t b = threadIdx.x; b < nbins; b += kSortBlock) row[b] = h[b];
@@ -1305,16 +1305,16 @@ template <class T>
 hipError_t launch_sort_chunked(const Soa<T> &in, const int32_t *pid_in, Soa<T> out, int32_t *pid_out, uint32_t np,
                                Geo g, const int32_t nb[3], const int32_t bd[3], int periodic, int32_t nrows,
                                uint32_t *keys, uint32_t *hist, uint32_t *counts, int32_t *start,
-                               uint32_t *cursor, unsigned int *bad, const unsigned int *gate, hipStream_t s) {
+                               uint32_t *cursor, const unsigned int *gate, hipStream_t s) {
     const int32_t nbins = nb[0] * nb[1] * nb[2];
     const uint32_t chunk = (np + (uint32_t)nrows - 1) / (uint32_t)nrows;
     const size_t lds = (size_t)nbins * 4;
     if (periodic)
         k_sort_count<T, true><<<nrows, kSortBlock, lds, s>>>(in, np, chunk, g, nb[0], nb[1], bd[0], bd[1], bd[2],
-                                                             nbins, keys, hist, bad, gate);
+                                                             nbins, keys, hist, gate);
     else
         k_sort_count<T, false><<<nrows, kSortBlock, lds, s>>>(in, np, chunk, g, nb[0], nb[1], bd[0], bd[1], bd[2],
-                                                              nbins, keys, hist, bad, gate);
+                                                              nbins, keys, hist, gate);
     k_col_prefix<<<grid_for((size_t)nbins, 256), 256, 0, s>>>(hist, nbins, nrows, counts, gate);
     k_scan<<<1, 1024, 0, s>>>(counts, nbins, start, cursor, gate);
     if (np) k_sort_scatter<T><<<nrows, kSortBlock, lds, s>>>(in, pid_in, out, pid_out, keys, hist, start, nbins, np,
@@ -1323,12 +1323,12 @@ hipError_t launch_sort_chunked(const Soa<T> &in, const int32_t *pid_in, Soa<T> o
 }
 template hipError_t launch_sort_chunked<float>(const Soa<float> &, const int32_t *, Soa<float>, int32_t *, uint32_t,
                                                Geo, const int32_t[3], const int32_t[3], int, int32_t, uint32_t *,
-                                               uint32_t *, uint32_t *, int32_t *, uint32_t *, unsigned int *,
+                                               uint32_t *, uint32_t *, int32_t *, uint32_t *,
                                                const unsigned int *, hipStream_t);
 template hipError_t launch_sort_chunked<double>(const Soa<double> &, const int32_t *, Soa<double>, int32_t *,
                                                 uint32_t, Geo, const int32_t[3], const int32_t[3], int, int32_t,
                                                 uint32_t *, uint32_t *, uint32_t *, int32_t *, uint32_t *,
-                                                unsigned int *, const unsigned int *, hipStream_t);
+                                                const unsigned int *, hipStream_t);
 
 
 template <class T>

@@ -167,8 +167,9 @@ int st_autotune_batch(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field 
     const int per = p.substeps_per_pass > 0 ? p.substeps_per_pass : kTuneSubsteps;
     const int passes = (kTuneSubsteps + per - 1) / per;
     const int ntrials = std::max(trials, kMinTrials);
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
+    hipEvent_t ev[2] = {nullptr, nullptr};  // timing only: no system-scope fence (as the pass timer's)
+    if (hipEventCreateWithFlags(&ev[0], hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&ev[1], hipEventDisableSystemFence) != hipSuccess) {
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         snap_free(sn);
