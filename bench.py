@@ -315,7 +315,8 @@ def channel_workload(stk, inputs, ctx, steps=5, warmup=1, np_parcels=1 << 25, in
     chunk.close()
     field.close()
     updates = sum(n0 + (k + 1) * inject_per_step for k in range(steps)) * case.nsub
-    bpp = bytes_per_pass((n0 + n1) // 2, case.ncell, True)
+    # parcels each timed step's passes sweep: n0 plus the step's own and earlier injections
+    bpp = bytes_per_pass(n0 + inject_per_step * (steps + 1) // 2, case.ncell, True)
     return {"parcels_start": n0, "parcels_end": n1, "removed_total": removed, "inject_per_step": inject_per_step,
             "cells": list(case.n), "substeps_per_step": case.nsub, "updates_per_s": updates / el,
             "ms_per_step": 1e3 * el / steps, "kernel_ms_per_pass": kms,
