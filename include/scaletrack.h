@@ -273,7 +273,9 @@ int st_autotune_batch(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field 
  * checkpoint_every substeps with (x, v) saved before each pass and recomputed in the
  * reverse sweep.  The derivative is the almost-everywhere one: cells, bounces and wraps
  * are held at the forward run's.  Sequence: create, forward, seed (source misfit or
- * cotangents), backward, gradients.  Host arrays are in upload order. */
+ * cotangents), backward, gradients; after one forward, seeding, backward and gradients
+ * may repeat (each backward starts from the seeds last given, which stay until the next
+ * forward).  Host arrays are in upload order. */
 typedef struct st_adjoint st_adjoint;
 int st_adjoint_create(st_ctx *ctx, st_cloud *cloud, st_field *field, int32_t nsub,
                       int32_t checkpoint_every, const st_params *params, st_adjoint **out);
@@ -291,6 +293,17 @@ int st_adjoint_backward(st_adjoint *adj);
  * grad_cd (scalar), grad_x0 and grad_v0 (SoA 3*n, gradients of the initial state). */
 int st_adjoint_gradients(st_adjoint *adj, double *grad_d, double *grad_rho, double *grad_cd,
                          double *grad_x0, double *grad_v0);
+
+/* ---- host memory -------------------------------------------------------------- */
+/* No counterpart in the reference, whose host arrays are the host language's own.
+ * Page-locked host memory, which every entry point taking host arrays copies to and from
+ * by DMA at the bus's rate (pageable arrays are staged through the runtime's bounce
+ * buffers at a fraction of it).  Allocating pins the pages and costs more than one
+ * transfer of the same size: allocate once and reuse.  st_host_alloc returns
+ * ST_E_INVALID for bytes < 1 or a null out, ST_E_NOMEM when the allocation fails;
+ * st_host_free(NULL) does nothing. */
+int st_host_alloc(int64_t bytes, void **out);
+int st_host_free(void *ptr);
 
 /* ---- diagnostics -------------------------------------------------------------- */
 /* No counterpart in the reference.  Streams `bytes` from one device buffer to another

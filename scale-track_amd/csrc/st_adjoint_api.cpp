@@ -25,9 +25,10 @@ struct st_adjoint {
     bool backward_done = false;
     int64_t epoch = -1;  // the cloud's epoch right after the forward
     double *ck = nullptr;    // nseg * 6 * n
-    double *xb = nullptr;    // 3n
-    double *vb = nullptr;    // 3n, propagated by the backward sweep
-    double *vseed = nullptr; // 3n, the final-velocity cotangent the sweep starts from
+    double *xb = nullptr;    // 3n, propagated by the backward sweep (then dL/dx0)
+    double *vb = nullptr;    // 3n, propagated by the backward sweep (then dL/dv0)
+    double *xseed = nullptr; // 3n, the final-position cotangent each sweep starts from
+    double *vseed = nullptr; // 3n, the final-velocity cotangent each sweep starts from
     double *fbar = nullptr;  // n
     double *mbar = nullptr;  // n
     double *G = nullptr;     // 3 * ncell
@@ -44,7 +45,7 @@ int destroy(st_adjoint *a) {
     quiesce_ctx(a->ctx);
     if (a->cloud) unpin(a->cloud);
     if (a->field) unpin(a->field);
-    for (double *q : {a->ck, a->xb, a->vb, a->vseed, a->fbar, a->mbar, a->G, a->UG, a->T, a->tmp, a->scal})
+    for (double *q : {a->ck, a->xb, a->vb, a->xseed, a->vseed, a->fbar, a->mbar, a->G, a->UG, a->T, a->tmp, a->scal})
         if (q) (void)hipFree(q);
     st_ctx *ctx = a->ctx;
     delete a;
@@ -100,6 +101,7 @@ int st_adjoint_create(st_ctx *ctx, st_cloud *c, st_field *f, int32_t nsub, int32
     hipError_t e = hipMalloc((void **)&a->ck, (size_t)a->nseg * 6 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->xb, 3 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->vb, 3 * n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->xseed, 3 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->vseed, 3 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->fbar, n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->mbar, n * 8);
@@ -154,7 +156,7 @@ int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
             ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false, sb, false));
         }
     }
-    ST_HIP(st::launch_fill_f64(a->xb, 0.0, 3 * (size_t)n, s));
+    ST_HIP(st::launch_fill_f64(a->xseed, 0.0, 3 * (size_t)n, s));
     ST_HIP(st::launch_fill_f64(a->vseed, 0.0, 3 * (size_t)n, s));
     ST_HIP(st::launch_fill_f64(a->G, 0.0, (size_t)f->ncell * 3, s));
     a->forward_done = true;
@@ -199,7 +201,7 @@ int st_adjoint_set_state_cotangent(st_adjoint *a, const double *xbar, const doub
     const int32_t *pid = a->cloud->pid[a->cloud->cur];
     if (xbar) {
         ST_HIP(hipMemcpyAsync(a->tmp, xbar, 3 * (size_t)n * 8, hipMemcpyHostToDevice, s));
-        ST_HIP(st::launch_gather_pid(a->tmp, pid, a->xb, n, 3, s));
+        ST_HIP(st::launch_gather_pid(a->tmp, pid, a->xseed, n, 3, s));
     }
     if (vbar) {
         ST_HIP(hipMemcpyAsync(a->tmp + 3 * (size_t)n, vbar, 3 * (size_t)n * 8, hipMemcpyHostToDevice, s));
@@ -219,6 +221,9 @@ int st_adjoint_backward(st_adjoint *a) {
     const uint32_t n = a->n;
     ST_HIP(st::launch_fill_f64(a->fbar, 0.0, n, s));
     ST_HIP(st::launch_fill_f64(a->mbar, 0.0, n, s));
+    // the sweep overwrites xb and vb: each backward starts again from the seeds, so one
+    // forward serves several losses
+    if (n) ST_HIP(hipMemcpyAsync(a->xb, a->xseed, 3 * (size_t)n * 8, hipMemcpyDeviceToDevice, s));
     if (n) ST_HIP(hipMemcpyAsync(a->vb, a->vseed, 3 * (size_t)n * 8, hipMemcpyDeviceToDevice, s));
     st::Soa<double> cur = soa<double>(c, c->cur);
     st::AdjArgsHost h{};
@@ -271,16 +276,14 @@ int st_adjoint_gradients(st_adjoint *a, double *grad_d, double *grad_rho, double
         for (int k = 0; k < 3; ++k) items[ni++] = {a->xb + (size_t)k * n, grad_x0 + (size_t)k * n};
     if (grad_v0)
         for (int k = 0; k < 3; ++k) items[ni++] = {a->vb + (size_t)k * n, grad_v0 + (size_t)k * n};
+    // one staging array reused in stream order: each scatter follows the previous copy
     for (int i = 0; i < ni; ++i) {
         if (!items[i].dst || !n) continue;
         ST_HIP(st::launch_scatter_pid(items[i].src, pid, stage, n, s));
         ST_HIP(hipMemcpyAsync(items[i].dst, stage, (size_t)n * 8, hipMemcpyDeviceToHost, s));
-        ST_HIP(hipStreamSynchronize(s));
     }
-    if (grad_cd) {
-        ST_HIP(hipMemcpyAsync(grad_cd, a->scal + 1, 8, hipMemcpyDeviceToHost, s));
-        ST_HIP(hipStreamSynchronize(s));
-    }
+    if (grad_cd) ST_HIP(hipMemcpyAsync(grad_cd, a->scal + 1, 8, hipMemcpyDeviceToHost, s));
+    ST_HIP(hipStreamSynchronize(s));
     return ST_OK;
 }
 

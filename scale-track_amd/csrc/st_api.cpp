@@ -1292,6 +1292,23 @@ int st_diag_kernel_times(st_ctx *ctx, int32_t max, double *ms, int32_t *launches
     return ST_OK;
 }
 
+int st_host_alloc(int64_t bytes, void **out) {
+    if (out) *out = nullptr;
+    if (!out || bytes < 1) return fail(ST_E_INVALID, "need bytes >= 1 and an output");
+    const hipError_t e = hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(ST_E_NOMEM, "page-locked host allocation of %lld B: %s", (long long)bytes,
+                    hipGetErrorString(e));
+    }
+    return ST_OK;
+}
+
+int st_host_free(void *ptr) {
+    if (ptr) ST_HIP(hipHostFree(ptr));
+    return ST_OK;
+}
+
 int st_diag_copy_bandwidth(st_ctx *ctx, int64_t bytes, int32_t reps, double *gbs) {
     ST_TRY(check_ctx(ctx));
     if (!gbs || bytes < 16 || reps < 1) return fail(ST_E_INVALID, "need bytes >= 16, reps >= 1 and an output");

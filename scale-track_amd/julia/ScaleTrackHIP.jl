@@ -361,10 +361,12 @@ set_state_cotangent!(a::Adjoint, xbar, vbar) =
     check(ccall((:st_adjoint_set_state_cotangent, lib), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), a.h,
                 xbar === nothing ? Ptr{Float64}(C_NULL) : xbar, vbar === nothing ? Ptr{Float64}(C_NULL) : vbar))
 backward!(a::Adjoint) = check(ccall((:st_adjoint_backward, lib), Cint, (Ptr{Cvoid},), a.h))
-"""gradients(a; per_parcel_rho) -- (dL/dd, dL/dρ or nothing, dL/dc_D)."""
-function gradients(a::Adjoint; per_parcel_rho::Bool=false)
-    gd = zeros(a.n)
-    gρ = per_parcel_rho ? zeros(a.n) : nothing
+"""gradients(a; per_parcel_rho, gd, gρ) -- (dL/dd, dL/dρ or nothing, dL/dc_D); `gd` and
+`gρ` may be preallocated (for instance with `pinned_array`) and are filled in place."""
+function gradients(a::Adjoint; per_parcel_rho::Bool=false, gd::Vector{Float64}=zeros(a.n),
+                   gρ::Union{Vector{Float64},Nothing}=per_parcel_rho ? zeros(a.n) : nothing)
+    length(gd) == a.n || throw(ArgumentError("gd must have $(a.n) elements"))
+    gρ === nothing || length(gρ) == a.n || throw(ArgumentError("gρ must have $(a.n) elements"))
     gc = Ref{Float64}(0.0)
     check(ccall((:st_adjoint_gradients, lib), Cint,
                 (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ref{Float64}, Ptr{Float64}, Ptr{Float64}),
@@ -376,6 +378,19 @@ function destroy!(a::Adjoint)
     a.h = C_NULL
     a.keep = nothing
     nothing
+end
+
+"""pinned_array(T, dims...) -- an uninitialised Array on page-locked host memory
+(st_host_alloc), freed by the library when the array is collected.  Copies between it
+and the device run by DMA at the bus's rate; pinning costs more than one transfer of
+the same size, so allocate once and reuse."""
+function pinned_array(::Type{T}, dims::Integer...) where {T}
+    p = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:st_host_alloc, lib), Cint, (Int64, Ref{Ptr{Cvoid}}), max(prod(dims) * sizeof(T), 1), p))
+    a = unsafe_wrap(Array, Ptr{T}(p[]), Tuple(dims); own=false)
+    ptr = p[]
+    finalizer(_ -> ccall((:st_host_free, lib), Cint, (Ptr{Cvoid},), ptr), a)
+    a
 end
 
 """source_misfit(ctx, c, f, target, Δt; nsub, K, p) -- L = Σ (UTrans - target)², the

@@ -6,7 +6,7 @@ from . import _lib  # noqa: F401  (loads the library)
 from .tracking import (  # noqa: F401
     CHANNEL, FP32, FP64, FLUSH_ATOMIC, FLUSH_OWNER, PERIODIC, PLAIN, SORTED, WALL, Chunk, Context, Graph,
     Batch, Mesh, Time, TwoWayEulerian, construct_mesh, device_count, evolve, evolve_device,
-    autotune_batch, increment_time, params)
+    autotune_batch, increment_time, params, pinned_copy, pinned_empty)
 from ._lib import ScaleTrackError  # noqa: F401
 
 __version__ = "0.1.0"

@@ -113,6 +113,8 @@ SIGNATURES = {
     "st_graph_launch": (C.c_int, [_vp]),
     "st_graph_destroy": (C.c_int, [_vp]),
     "st_last_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "st_host_alloc": (C.c_int, [C.c_int64, _pp]),
+    "st_host_free": (C.c_int, [_vp]),
     "st_diag_copy_bandwidth": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "st_diag_kernel_times": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_int32)]),

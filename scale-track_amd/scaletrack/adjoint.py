@@ -54,13 +54,26 @@ class Adjoint:
     def backward(self):
         check(lib.st_adjoint_backward(self.handle))
 
-    def gradients(self, initial_state=False):
+    def gradients(self, initial_state=False, out=None):
+        """Per-parcel gradients in upload order.  `out` may hold preallocated arrays
+        ("d", "rho", "x0", "v0"; float64, C-contiguous, of shapes (n,) and (3, n)), for
+        instance page-locked ones from `pinned_empty`, which the download fills."""
         n = self.chunk.N
-        gd = np.empty(n)
-        grho = np.empty(n) if self.chunk.rho_p is not None else None
+        out = {} if out is None else out
+
+        def dst(key, shape):
+            a = out.get(key)
+            if a is None:
+                return np.empty(shape)
+            if a.shape != shape or a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError(f"out[{key!r}] must be a C-contiguous float64 array of shape {shape}")
+            return a
+
+        gd = dst("d", (n,))
+        grho = dst("rho", (n,)) if self.chunk.rho_p is not None else None
         gcd = C.c_double()
-        gx = np.empty((3, n)) if initial_state else None
-        gv = np.empty((3, n)) if initial_state else None
+        gx = dst("x0", (3, n)) if initial_state else None
+        gv = dst("v0", (3, n)) if initial_state else None
         check(lib.st_adjoint_gradients(self.handle, _ptr(gd), _ptr(grho), C.byref(gcd), _ptr(gx), _ptr(gv)))
         out = {"d": gd, "rho": grho, "cd": gcd.value}
         if initial_state:

@@ -30,6 +30,41 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+class _PinnedBlock:
+    """One st_host_alloc block exposed through the array interface; freed when the last
+    array viewing it goes."""
+
+    def __init__(self, shape, dtype):
+        dt = np.dtype(dtype)
+        shape = tuple(int(k) for k in (shape if np.iterable(shape) else (shape,)))
+        nbytes = int(np.prod(shape, dtype=np.int64)) * dt.itemsize
+        p = C.c_void_p()
+        check(lib.st_host_alloc(max(nbytes, 1), C.byref(p)))
+        self.ptr = p
+        self.__array_interface__ = {"shape": shape, "typestr": dt.str, "data": (p.value, False), "version": 3}
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            lib.st_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype=np.float64):
+    """An uninitialised C-contiguous numpy array on page-locked host memory
+    (st_host_alloc).  Copies between it and the device run by DMA at the bus's rate
+    instead of through the runtime's bounce buffers.  Pinning costs more than one
+    transfer of the same size: allocate once and reuse."""
+    return np.asarray(_PinnedBlock(shape, dtype))
+
+
+def pinned_copy(a):
+    """`a` copied into a new page-locked array of the same shape and dtype."""
+    a = np.asarray(a)
+    out = pinned_empty(a.shape, a.dtype)
+    out[...] = a
+    return out
+
+
 def device_count():
     n = C.c_int32(0)
     check(lib.st_device_count(C.byref(n)))

@@ -71,6 +71,12 @@ def test_null_arguments_are_rejected():
     # destroying null handles is a no-op
     for fn in ("st_ctx_destroy", "st_cloud_destroy", "st_field_destroy", "st_graph_destroy"):
         assert getattr(lib, fn)(None) == _lib.ST_OK
+    # page-locked host memory: sizes below one byte and a null output are refused
+    p = C.c_void_p(1234)
+    assert lib.st_host_alloc(0, C.byref(p)) == _lib.ST_E_INVALID and not p.value
+    assert lib.st_host_alloc(-8, C.byref(p)) == _lib.ST_E_INVALID
+    assert lib.st_host_alloc(64, None) == _lib.ST_E_INVALID
+    assert lib.st_host_free(None) == _lib.ST_OK
 
 
 def test_no_device_is_a_clean_error():

@@ -188,3 +188,49 @@ def test_adjoint_forward_resorts_a_cloud_only_it_holds(ctx):
     for k in ("d", "rho", "cd"):
         a, b = np.asarray(g0[k], float), np.asarray(g1[k], float)
         assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max(), k
+
+
+def test_gradients_into_page_locked_arrays(ctx):
+    """The target and the gradients on page-locked host arrays (st_host_alloc through
+    pinned_empty / pinned_copy), reused over two runs, the first with two losses (two
+    backward sweeps) after one forward.  After the same forward every per-parcel gradient
+    equals, bit for bit, that of pageable numpy arrays after a single sweep (the loss and
+    dL/dcd, sums taken with atomics, to 1e-12); after a second forward, whose source sums
+    add in another order, to 1e-9.  A mis-shaped output is refused before anything is
+    copied."""
+    case = make_case(np_parcels=3000, boundary=1, nsub=20)
+    T = target(case)
+    Tp = stk.pinned_copy(T)
+    assert np.array_equal(Tp, T) and Tp.flags.c_contiguous
+    p = stk.params(boundary=1, brick=(4, 4, 4))
+    field, chunk = gpu_objects(ctx, case)
+    adj = Adjoint(ctx, chunk, field, case.nsub, 5, p)
+    n = case.np
+    out = {"d": stk.pinned_empty(n), "rho": stk.pinned_empty(n), "x0": stk.pinned_empty((3, n)),
+           "v0": stk.pinned_empty((3, n))}
+    ref = None
+    for run in range(2):
+        chunk.set_state(*case.state())
+        chunk.upload()
+        adj.forward(case.dt_fluid)
+        if ref is None:
+            loss = adj.source_misfit(T)
+            adj.backward()
+            ref = (loss, adj.gradients(initial_state=True))
+        loss_p = adj.source_misfit(Tp)
+        adj.backward()
+        g = adj.gradients(initial_state=True, out=out)
+        tol = 1e-12 if run == 0 else 1e-9
+        assert abs(loss_p - ref[0]) <= tol * abs(ref[0])
+        for k in ("d", "rho", "x0", "v0"):
+            assert g[k] is out[k]
+            if run == 0:
+                assert np.array_equal(g[k], ref[1][k]), k
+            else:
+                assert np.abs(g[k] - ref[1][k]).max() <= tol * np.abs(ref[1][k]).max(), k
+        assert abs(g["cd"] - ref[1]["cd"]) <= tol * abs(ref[1]["cd"])
+    with pytest.raises(ValueError):
+        adj.gradients(out={"d": stk.pinned_empty(n - 1)})
+    adj.close()
+    chunk.close()
+    field.close()
