@@ -29,9 +29,13 @@ struct AdjArgs {
     const double *U;        // cell velocity, AoS
     const double *G;        // dL/dS, AoS
     const double *UG;       // per cell U xyz then G xyz (48 B, 16-byte aligned)
-    double *xb;             // adjoint of x, y, z (SoA, 3n)
-    double *vb;             // adjoint of u, v, w (SoA, 3n)
+    const double *xb;       // adjoint of x, y, z (SoA, 3n; constant through the substeps), or
+                            // nullptr for zero
+    const double *vb_in;    // adjoint of u, v, w the segment starts from (SoA, 3n), or nullptr
+                            // for zero
+    double *vb;             // adjoint of u, v, w the segment ends with (may equal vb_in)
     double *fbar, *mbar;    // accumulated adjoints of f and m per parcel
+    int first;              // the sweep's first segment: fbar and mbar start at zero
     Geo g;
     Phys<double> ph;
     int32_t K;              // substeps in this segment
@@ -74,8 +78,17 @@ k_adjoint_segment(AdjArgs a) {
     const double rden = 1.0 / den;
     const Quot<double> qden(den);
 
-    double xb0 = a.xb[p], xb1 = a.xb[n + p], xb2 = a.xb[2 * n + p];
-    double vb0 = a.vb[p], vb1 = a.vb[n + p], vb2 = a.vb[2 * n + p];
+    double xb0 = 0.0, xb1 = 0.0, xb2 = 0.0, vb0 = 0.0, vb1 = 0.0, vb2 = 0.0;
+    if (a.xb) {
+        xb0 = a.xb[p];
+        xb1 = a.xb[n + p];
+        xb2 = a.xb[2 * n + p];
+    }
+    if (a.vb_in) {
+        vb0 = a.vb_in[p];
+        vb1 = a.vb_in[n + p];
+        vb2 = a.vb_in[2 * n + p];
+    }
     double fb = 0.0, mb = 0.0;  // fb without its factor 1 / (1 + f), applied at the end
 
     const double cx = a.ck[p], cy = a.ck[n + p], cz = a.ck[2 * n + p];
@@ -216,8 +229,8 @@ k_adjoint_segment(AdjArgs a) {
     a.vb[p] = vb0;
     a.vb[n + p] = vb1;
     a.vb[2 * n + p] = vb2;
-    a.fbar[p] += fb * rden;
-    a.mbar[p] += mb;
+    a.fbar[p] = (a.first ? 0.0 : a.fbar[p]) + fb * rden;
+    a.mbar[p] = (a.first ? 0.0 : a.mbar[p]) + mb;
 }
 
 struct GradArgs {
@@ -310,9 +323,9 @@ __global__ void k_interleave3(const double *a, const double *b, double *dst, siz
         q[0] = a[3 * c];
         q[1] = a[3 * c + 1];
         q[2] = a[3 * c + 2];
-        q[3] = b[3 * c];
-        q[4] = b[3 * c + 1];
-        q[5] = b[3 * c + 2];
+        q[3] = b ? b[3 * c] : 0.0;  // b == nullptr: zero
+        q[4] = b ? b[3 * c + 1] : 0.0;
+        q[5] = b ? b[3 * c + 2] : 0.0;
     }
 }
 
@@ -332,7 +345,9 @@ hipError_t launch_adjoint_segment(const AdjArgsHost &h, hipStream_t s) {
     a.G = h.G;
     a.UG = h.UG;
     a.xb = h.xb;
+    a.vb_in = h.vb_in;
     a.vb = h.vb;
+    a.first = h.first ? 1 : 0;
     a.fbar = h.fbar;
     a.mbar = h.mbar;
     a.g = h.g;

@@ -7,6 +7,7 @@
 // diameter and density and to the drag-coefficient multiplier.
 #include "st_handles.h"
 
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <vector>
@@ -23,11 +24,14 @@ struct st_adjoint {
     uint32_t n = 0;
     bool forward_done = false;
     bool backward_done = false;
+    // seeds given since the forward; one not given is zero, and the sweep reads nullptr
+    // in its place (no fill in the forward, no read in every segment)
+    bool xb_set = false, vseed_set = false, G_set = false;
     int64_t epoch = -1;  // the cloud's epoch right after the forward
     double *ck = nullptr;    // nseg * 6 * n
-    double *xb = nullptr;    // 3n, propagated by the backward sweep (then dL/dx0)
+    double *xb = nullptr;    // 3n, the final-position cotangent; the sweep only reads it (the
+                             // adjoint of x is constant through the substeps, so it is dL/dx0)
     double *vb = nullptr;    // 3n, propagated by the backward sweep (then dL/dv0)
-    double *xseed = nullptr; // 3n, the final-position cotangent each sweep starts from
     double *vseed = nullptr; // 3n, the final-velocity cotangent each sweep starts from
     double *fbar = nullptr;  // n
     double *mbar = nullptr;  // n
@@ -45,7 +49,7 @@ int destroy(st_adjoint *a) {
     quiesce_ctx(a->ctx);
     if (a->cloud) unpin(a->cloud);
     if (a->field) unpin(a->field);
-    for (double *q : {a->ck, a->xb, a->vb, a->xseed, a->vseed, a->fbar, a->mbar, a->G, a->UG, a->T, a->tmp, a->scal})
+    for (double *q : {a->ck, a->xb, a->vb, a->vseed, a->fbar, a->mbar, a->G, a->UG, a->T, a->tmp, a->scal})
         if (q) (void)hipFree(q);
     st_ctx *ctx = a->ctx;
     delete a;
@@ -101,7 +105,6 @@ int st_adjoint_create(st_ctx *ctx, st_cloud *c, st_field *f, int32_t nsub, int32
     hipError_t e = hipMalloc((void **)&a->ck, (size_t)a->nseg * 6 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->xb, 3 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->vb, 3 * n * 8);
-    if (e == hipSuccess) e = hipMalloc((void **)&a->xseed, 3 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->vseed, 3 * n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->fbar, n * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->mbar, n * 8);
@@ -156,9 +159,7 @@ int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
             ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false, sb, false));
         }
     }
-    ST_HIP(st::launch_fill_f64(a->xseed, 0.0, 3 * (size_t)n, s));
-    ST_HIP(st::launch_fill_f64(a->vseed, 0.0, 3 * (size_t)n, s));
-    ST_HIP(st::launch_fill_f64(a->G, 0.0, (size_t)f->ncell * 3, s));
+    a->xb_set = a->vseed_set = a->G_set = false;
     a->forward_done = true;
     a->backward_done = false;
     a->epoch = c->epoch;
@@ -176,6 +177,7 @@ int st_adjoint_source_misfit(st_adjoint *a, const double *target, double *loss) 
     hipError_t e = hipMemcpyAsync(a->T, target, nc3 * 8, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = st::launch_fill_f64(a->scal, 0.0, 1, s);
     if (e == hipSuccess) e = st::launch_misfit(f->src, a->T, a->G, a->scal, nc3, s);
+    if (e == hipSuccess) a->G_set = true;
     if (e == hipSuccess) e = hipMemcpyAsync(loss, a->scal, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail(ST_E_DEVICE, "source misfit: %s", hipGetErrorString(e));
@@ -189,6 +191,7 @@ int st_adjoint_set_source_cotangent(st_adjoint *a, const double *g) {
     const size_t nc3 = (size_t)a->field->ncell * 3;
     ST_HIP(hipMemcpyAsync(a->G, g, nc3 * 8, hipMemcpyHostToDevice, a->ctx->stream));
     ST_HIP(hipStreamSynchronize(a->ctx->stream));
+    a->G_set = true;
     return ST_OK;
 }
 
@@ -201,11 +204,13 @@ int st_adjoint_set_state_cotangent(st_adjoint *a, const double *xbar, const doub
     const int32_t *pid = a->cloud->pid[a->cloud->cur];
     if (xbar) {
         ST_HIP(hipMemcpyAsync(a->tmp, xbar, 3 * (size_t)n * 8, hipMemcpyHostToDevice, s));
-        ST_HIP(st::launch_gather_pid(a->tmp, pid, a->xseed, n, 3, s));
+        ST_HIP(st::launch_gather_pid(a->tmp, pid, a->xb, n, 3, s));
+        a->xb_set = true;
     }
     if (vbar) {
         ST_HIP(hipMemcpyAsync(a->tmp + 3 * (size_t)n, vbar, 3 * (size_t)n * 8, hipMemcpyHostToDevice, s));
         ST_HIP(st::launch_gather_pid(a->tmp + 3 * (size_t)n, pid, a->vseed, n, 3, s));
+        a->vseed_set = true;
     }
     ST_HIP(hipStreamSynchronize(s));
     return ST_OK;
@@ -219,12 +224,10 @@ int st_adjoint_backward(st_adjoint *a) {
     st_field *f = a->field;
     hipStream_t s = a->ctx->stream;
     const uint32_t n = a->n;
-    ST_HIP(st::launch_fill_f64(a->fbar, 0.0, n, s));
-    ST_HIP(st::launch_fill_f64(a->mbar, 0.0, n, s));
-    // the sweep overwrites xb and vb: each backward starts again from the seeds, so one
-    // forward serves several losses
-    if (n) ST_HIP(hipMemcpyAsync(a->xb, a->xseed, 3 * (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-    if (n) ST_HIP(hipMemcpyAsync(a->vb, a->vseed, 3 * (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    // The first segment swept (the last in time) reads the velocity cotangent from the
+    // seed and starts fbar and mbar at zero; the later ones carry vb, fbar and mbar on.
+    // So each backward starts again from the seeds (one forward serves several losses),
+    // with no copy or fill in front of it.
     st::Soa<double> cur = soa<double>(c, c->cur);
     st::AdjArgsHost h{};
     h.d = cur.d;
@@ -232,8 +235,8 @@ int st_adjoint_backward(st_adjoint *a) {
     h.U = (const double *)f->U;
     h.G = a->G;
     h.UG = a->UG;
-    ST_HIP(st::launch_interleave3(h.U, a->G, a->UG, (size_t)f->ncell, s));
-    h.xb = a->xb;
+    ST_HIP(st::launch_interleave3(h.U, a->G_set ? a->G : nullptr, a->UG, (size_t)f->ncell, s));
+    h.xb = a->xb_set ? a->xb : nullptr;
     h.vb = a->vb;
     h.fbar = a->fbar;
     h.mbar = a->mbar;
@@ -245,6 +248,8 @@ int st_adjoint_backward(st_adjoint *a) {
     for (int32_t seg = a->nseg - 1; seg >= 0; --seg) {
         h.ck = a->ck + (size_t)seg * 6 * n;
         h.K = std::min(a->K, a->nsub - seg * a->K);
+        h.first = seg == a->nseg - 1;
+        h.vb_in = !h.first ? a->vb : a->vseed_set ? a->vseed : nullptr;
         ST_HIP(st::launch_adjoint_segment(h, s));
     }
     ST_HIP(timer_stop(a->ctx, s, a->nseg));
@@ -272,8 +277,10 @@ int st_adjoint_gradients(st_adjoint *a, double *grad_d, double *grad_rho, double
         double *dst;
     } items[8] = {{a->tmp, grad_d}, {a->tmp + n, grad_rho}};
     int ni = 2;
-    if (grad_x0)
+    if (grad_x0 && a->xb_set)
         for (int k = 0; k < 3; ++k) items[ni++] = {a->xb + (size_t)k * n, grad_x0 + (size_t)k * n};
+    else if (grad_x0)  // dL/dx0 is the final-position cotangent, and none was given
+        std::fill(grad_x0, grad_x0 + 3 * (size_t)n, 0.0);
     if (grad_v0)
         for (int k = 0; k < 3; ++k) items[ni++] = {a->vb + (size_t)k * n, grad_v0 + (size_t)k * n};
     // one staging array reused in stream order: each scatter follows the previous copy

@@ -197,7 +197,11 @@ hipError_t launch_copy(const void *a, void *b, size_t bytes, hipStream_t s);
 // Adjoint (st_adjoint.hip), FP64.
 struct AdjArgsHost {
     const double *d, *rho, *ck, *U, *G, *UG;
-    double *xb, *vb, *fbar, *mbar;
+    const double *xb;     // the final-position cotangent, or nullptr for zero
+    const double *vb_in;  // the velocity cotangent the segment starts from (vseed or vb), or
+                          // nullptr for zero
+    double *vb, *fbar, *mbar;
+    bool first;           // fbar and mbar start at zero (not read)
     Geo g;
     Phys<double> ph;
     int32_t K;
@@ -205,7 +209,7 @@ struct AdjArgsHost {
     int periodic;
 };
 hipError_t launch_adjoint_segment(const AdjArgsHost &h, hipStream_t s);
-// dst[6c .. 6c+5] = a[3c .. 3c+2], b[3c .. 3c+2] for c < n.
+// dst[6c .. 6c+5] = a[3c .. 3c+2], b[3c .. 3c+2] for c < n (b == nullptr: zeros).
 hipError_t launch_interleave3(const double *a, const double *b, double *dst, size_t n, hipStream_t s);
 hipError_t launch_adjoint_grads(const double *d, const double *rho, const double *fbar, const double *mbar,
                                 double *gd, double *grho, double *gcd, const Phys<double> &ph, uint32_t np,
