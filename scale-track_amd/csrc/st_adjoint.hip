@@ -237,18 +237,42 @@ struct GradArgs {
     const double *d, *rho;
     const double *fbar, *mbar;
     double *gd, *grho;  // per parcel (grho null with a chunk density)
-    double *gcd;        // scalar, accumulated
+    double *part;       // one partial of dL/dcd per workgroup
     Phys<double> ph;
     uint32_t np;
 };
 
 // d f/d d = -2 f / d, d f/d rho = -f / rho, d f/d cd = f / cd (= the factor without cd),
 // d m/d d = 3 m / d, d m/d rho = m / rho (per-parcel density only).
-__global__ void __launch_bounds__(256) k_adjoint_grads(GradArgs a) {
+__device__ __forceinline__ void block_partial(double v, double *part) {
     __shared__ double red[256 / kWave];
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s += red[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+// out = sum of part[0 .. n), n <= kSumBlocks, in a fixed order (one workgroup).
+__global__ void __launch_bounds__(kSumBlocks) k_sum_partials(const double *part, int n, double *out) {
+    __shared__ double red[kSumBlocks / kWave];
+    double v = (int)threadIdx.x < n ? part[threadIdx.x] : 0.0;
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kSumBlocks / kWave; ++w) s += red[w];
+        out[0] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_adjoint_grads(GradArgs a) {
     double gc = 0.0;
-    if (p < a.np) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < a.np; p += gridDim.x * blockDim.x) {
         const double dia = a.d[p];
         const double rho = a.rho ? a.rho[p] : a.ph.rho;
         double m, f, den;
@@ -257,36 +281,21 @@ __global__ void __launch_bounds__(256) k_adjoint_grads(GradArgs a) {
         const double fb = a.fbar[p], mb = a.mbar[p];
         a.gd[p] = fb * (-2.0 * f / dia) + mb * (3.0 * m / dia);
         if (a.grho) a.grho[p] = fb * (-f / rho) + mb * (m / rho);
-        gc = fb * fref;
+        gc += fb * fref;
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) gc += __shfl_xor(gc, off, kWave);
-    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = gc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s += red[w];
-        atomic_add_global(a.gcd, s);
-    }
+    block_partial(gc, a.part);
 }
 
 // G = 2 (S - T) and L += sum (S - T)^2 (the source-misfit loss).
-__global__ void __launch_bounds__(256) k_misfit(const double *S, const double *T, double *G, double *loss,
+__global__ void __launch_bounds__(256) k_misfit(const double *S, const double *T, double *G, double *part,
                                                 size_t n) {
-    __shared__ double red[256 / kWave];
     double acc = 0.0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const double r = S[i] - T[i];
         G[i] = 2.0 * r;
         acc += r * r;
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
-    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s += red[w];
-        atomic_add_global(loss, s);
-    }
+    block_partial(acc, part);
 }
 
 // dst[k*n + p] = src[k][p] for the six state arrays (a checkpoint).
@@ -364,19 +373,22 @@ hipError_t launch_adjoint_segment(const AdjArgsHost &h, hipStream_t s) {
 }
 
 hipError_t launch_adjoint_grads(const double *d, const double *rho, const double *fbar, const double *mbar,
-                                double *gd, double *grho, double *gcd, const Phys<double> &ph, uint32_t np,
-                                hipStream_t s) {
-    if (!np) return hipSuccess;
-    GradArgs a{d, rho, fbar, mbar, gd, grho, gcd, ph, np};
-    k_adjoint_grads<<<(np + 255) / 256, 256, 0, s>>>(a);
+                                double *gd, double *grho, double *gcd, double *part, const Phys<double> &ph,
+                                uint32_t np, hipStream_t s) {
+    const int grid = (int)std::min<uint32_t>((np + 255) / 256, (uint32_t)kSumBlocks);
+    if (grid > 0) {
+        GradArgs a{d, rho, fbar, mbar, gd, grho, part, ph, np};
+        k_adjoint_grads<<<grid, 256, 0, s>>>(a);
+    }
+    k_sum_partials<<<1, kSumBlocks, 0, s>>>(part, grid, gcd);
     return hipGetLastError();
 }
 
-hipError_t launch_misfit(const double *S, const double *T, double *G, double *loss, size_t n, hipStream_t s) {
-    if (!n) return hipSuccess;
-    size_t g = (n + 255) / 256;
-    if (g > 4096) g = 4096;
-    k_misfit<<<(unsigned)g, 256, 0, s>>>(S, T, G, loss, n);
+hipError_t launch_misfit(const double *S, const double *T, double *G, double *loss, double *part, size_t n,
+                         hipStream_t s) {
+    const int grid = (int)std::min<size_t>((n + 255) / 256, (size_t)kSumBlocks);
+    if (grid > 0) k_misfit<<<grid, 256, 0, s>>>(S, T, G, part, n);
+    k_sum_partials<<<1, kSumBlocks, 0, s>>>(part, grid, loss);
     return hipGetLastError();
 }
 

@@ -39,7 +39,7 @@ struct st_adjoint {
     double *UG = nullptr;    // 6 * ncell: U and G of each cell side by side (reverse sweep)
     double *T = nullptr;     // 3 * ncell: target source of st_adjoint_source_misfit
     double *tmp = nullptr;   // 6n staging
-    double *scal = nullptr;  // [0] loss, [1] d L / d cd
+    double *scal = nullptr;  // [0] loss, [1] d L / d cd, [2, 2 + kSumBlocks) partial sums
 };
 
 namespace {
@@ -112,7 +112,7 @@ int st_adjoint_create(st_ctx *ctx, st_cloud *c, st_field *f, int32_t nsub, int32
     if (e == hipSuccess) e = hipMalloc((void **)&a->UG, 2 * nc3 * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->T, nc3 * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&a->tmp, 6 * n * 8);
-    if (e == hipSuccess) e = hipMalloc((void **)&a->scal, 2 * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&a->scal, (2 + st::kSumBlocks) * 8);
     if (e == hipSuccess) e = st::launch_fill_f64(a->G, 0.0, nc3, ctx->stream);
     if (e != hipSuccess) {
         destroy(a);
@@ -175,8 +175,7 @@ int st_adjoint_source_misfit(st_adjoint *a, const double *target, double *loss) 
     const size_t nc3 = (size_t)f->ncell * 3;
     ST_TRY(merge_source(f));
     hipError_t e = hipMemcpyAsync(a->T, target, nc3 * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = st::launch_fill_f64(a->scal, 0.0, 1, s);
-    if (e == hipSuccess) e = st::launch_misfit(f->src, a->T, a->G, a->scal, nc3, s);
+    if (e == hipSuccess) e = st::launch_misfit(f->src, a->T, a->G, a->scal, a->scal + 2, nc3, s);
     if (e == hipSuccess) a->G_set = true;
     if (e == hipSuccess) e = hipMemcpyAsync(loss, a->scal, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -253,10 +252,9 @@ int st_adjoint_backward(st_adjoint *a) {
         ST_HIP(st::launch_adjoint_segment(h, s));
     }
     ST_HIP(timer_stop(a->ctx, s, a->nseg));
-    ST_HIP(st::launch_fill_f64(a->scal + 1, 0.0, 1, s));
-    // gd -> tmp[0, n), grho -> tmp[n, 2n), device order
+    // gd -> tmp[0, n), grho -> tmp[n, 2n), device order; dL/dcd -> scal[1]
     ST_HIP(st::launch_adjoint_grads(cur.d, cur.rho, a->fbar, a->mbar, a->tmp, cur.rho ? a->tmp + n : nullptr,
-                                    a->scal + 1, h.ph, n, s));
+                                    a->scal + 1, a->scal + 2, h.ph, n, s));
     a->backward_done = true;
     return ST_OK;
 }

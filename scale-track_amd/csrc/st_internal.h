@@ -211,10 +211,15 @@ struct AdjArgsHost {
 hipError_t launch_adjoint_segment(const AdjArgsHost &h, hipStream_t s);
 // dst[6c .. 6c+5] = a[3c .. 3c+2], b[3c .. 3c+2] for c < n (b == nullptr: zeros).
 hipError_t launch_interleave3(const double *a, const double *b, double *dst, size_t n, hipStream_t s);
+// The two sums below (dL/dcd, the loss) are taken without atomics: at most kSumBlocks
+// workgroups each write a partial to part[], and one workgroup adds them in a fixed
+// order into the output (so they repeat bit for bit from run to run).
+constexpr int kSumBlocks = 1024;
 hipError_t launch_adjoint_grads(const double *d, const double *rho, const double *fbar, const double *mbar,
-                                double *gd, double *grho, double *gcd, const Phys<double> &ph, uint32_t np,
-                                hipStream_t s);
-hipError_t launch_misfit(const double *S, const double *T, double *G, double *loss, size_t n, hipStream_t s);
+                                double *gd, double *grho, double *gcd, double *part, const Phys<double> &ph,
+                                uint32_t np, hipStream_t s);
+hipError_t launch_misfit(const double *S, const double *T, double *G, double *loss, double *part, size_t n,
+                         hipStream_t s);
 hipError_t launch_save6(const double *const f[6], double *dst, uint32_t n, hipStream_t s);
 hipError_t launch_gather_pid(const double *src, const int32_t *pid, double *dst, uint32_t n, int k,
                              hipStream_t s);
