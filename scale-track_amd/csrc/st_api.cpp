@@ -430,6 +430,17 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     const int group = p.method == ST_METHOD_SORTED && !ck && tblocks == 0 ? std::max(1, p.passes_per_launch) : 1;
     const int grid = group > 1 ? multi_grid<T>(ctx, c, p, a.tile_cells) : 0;
     if (group > 1 && grid < 1) return fail(ST_E_RANGE, "the multi-pass kernel does not fit: block %d", p.block);
+    // A pending zero of the source (st_field_zero_source) is taken by the first launch
+    // when that is one sorted owner-flush pass over the bricks: every brick starts its
+    // own cells' sums from zero and stores all of them.  Every other launch reads `src`,
+    // so the zeros are written first.  A capture never meets one: the capture entry
+    // points write it before they begin.
+    const bool lazy = f->src_zero && !capturing && p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER &&
+                      group == 1;
+    if (f->src_zero && !lazy) {
+        if (capturing) return fail(ST_E_STATE, "a pending zero of the source inside a capture");
+        ST_TRY(materialize_zero(f));
+    }
 
     // Statistics (bounding box, strays) are taken in the last pass only.
     const int32_t nslots = stats_slots(c, p);
@@ -447,6 +458,8 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
         const int next = done + k * a.nsteps;
         a.part = next >= nsub ? sb.part : nullptr;
         a.ck = done == 0 ? ck : nullptr;
+        a.zero_src = lazy && done == 0 ? 1 : 0;
+        ta.zero_src = a.zero_src;
         if (k > 1) {
             ST_HIP(launch_pass_multi<T>(a, k, bnd, p.block, grid, s));
         } else if (tblocks > 0) {  // the bricks and the unsorted tail in one launch
@@ -462,6 +475,7 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
         passes += k;
         done = next;
     }
+    if (lazy) f->src_zero = false;
     // the recorded time covers every pass; per-pass figures divide by the pass count
     if (!capturing) ST_HIP(timer_stop(ctx, s, passes));
     bool resort = device_resort && p.method == ST_METHOD_SORTED && c->sorted && c->n > 0;
@@ -556,7 +570,16 @@ int check_tile(const st_ctx *ctx, int prec, const int32_t brick[3]) {
     return ST_OK;
 }
 
+int materialize_zero(st_field *f) {
+    if (f->src_zero) {
+        ST_HIP(launch_fill_f64(f->src, 0.0, (size_t)f->ncell * 3, f->ctx->stream));
+        f->src_zero = false;
+    }
+    return ST_OK;
+}
+
 int merge_source(st_field *f) {
+    ST_TRY(materialize_zero(f));
     if (f->stray_dirty) {
         ST_HIP(launch_merge(f->src, f->stray, (size_t)f->ncell * 3, f->ctx->stream));
         f->stray_dirty = false;
@@ -1079,7 +1102,10 @@ int st_field_upload_U(st_field *f, const void *U, int32_t elem_bytes) {
 int st_field_zero_source(st_field *f) {
     if (!f) return fail(ST_E_INVALID, "null field");
     ST_TRY(check_ctx(f->ctx));
-    ST_HIP(st::launch_fill_f64(f->src, 0.0, (size_t)f->ncell * 3, f->ctx->stream));
+    // The zeros of `src` are written by the next step's first sorted owner-flush pass,
+    // which then stores every cell (no fill launch in front of it); any other use of the
+    // source writes them first (materialize_zero).
+    f->src_zero = true;
     if (f->stray_dirty) ST_HIP(st::launch_fill_f64(f->stray, 0.0, (size_t)f->ncell * 3, f->ctx->stream));
     f->stray_dirty = false;
     return ST_OK;
@@ -1098,10 +1124,7 @@ int st_field_download_source(st_field *f, void *out, int32_t elem_bytes) {
     if (elem_bytes != 4 && elem_bytes != 8) return fail(ST_E_INVALID, "elem_bytes must be 4 or 8 (got %d)", elem_bytes);
     hipStream_t s = f->ctx->stream;
     const size_t n = (size_t)f->ncell * 3;
-    if (f->stray_dirty) {
-        ST_HIP(st::launch_merge(f->src, f->stray, n, s));
-        f->stray_dirty = false;
-    }
+    ST_TRY(merge_source(f));
     if (elem_bytes == 8) {
         ST_HIP(hipMemcpyAsync(out, f->src, n * 8, hipMemcpyDeviceToHost, s));
         ST_HIP(hipStreamSynchronize(s));
@@ -1183,6 +1206,7 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
     if (p.method == ST_METHOD_SORTED) ST_TRY(sort_buffers(c, c->nbricks, true));
     StatsBuf sb;
     ST_TRY(cloud_stats(c, stats_slots(c, p), sb));
+    ST_TRY(materialize_zero(f));  // a zero pending now belongs to no replay
     ST_HIP(hipStreamSynchronize(ctx->stream));
     st_graph *g = new (std::nothrow) st_graph;
     if (!g) return fail(ST_E_NOMEM, "host allocation failed");
@@ -1230,6 +1254,9 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
 int st_graph_launch(st_graph *g) {
     if (!g) return fail(ST_E_INVALID, "null graph");
     ST_TRY(check_ctx(g->ctx));
+    // the replay's kernels read and write `src` as it is in memory
+    if (g->field) ST_TRY(materialize_zero(g->field));
+    for (st_field *f : g->more_fields) ST_TRY(materialize_zero(f));
     ST_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
     if (g->owner_flush) {
         if (g->field) g->field->stray_dirty = true;

@@ -153,6 +153,9 @@ int sort_all(st_batch *b) {
 
 int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
     ST_TRY(table_for(b, dt_sub));
+    // the batch kernel reads `src` (under capture the zeros were written before it began)
+    if (!capturing)
+        for (st_field *f : b->fields) ST_TRY(materialize_zero(f));
     hipStream_t s = b->ctx->stream;
     const int per = b->p.substeps_per_pass > 0 ? b->p.substeps_per_pass : nsub;
     if (!capturing) ST_HIP(timer_start(b->ctx, s));
@@ -354,6 +357,7 @@ int st_batch_graph_capture(st_batch *b, double dt_fluid, int32_t nsub, int32_t z
     if (nsub < 1) return fail(ST_E_INVALID, "nsub must be >= 1");
     const double dt_sub = substep_dt(b->prec, dt_fluid, nsub);
     ST_TRY(table_for(b, dt_sub));  // table upload happens before the capture
+    for (st_field *f : b->fields) ST_TRY(materialize_zero(f));  // a zero pending now belongs to no replay
     st_graph *g = new (std::nothrow) st_graph;
     if (!g) return fail(ST_E_NOMEM, "host allocation failed");
     g->ctx = b->ctx;

@@ -131,6 +131,11 @@ struct st_field {
     double *stray = nullptr;
     bool U_set = false;
     bool stray_dirty = false;
+    // st_field_zero_source was called and nothing has written `src` since: the memory
+    // still holds the old sums.  The next sorted owner-flush pass starts every brick's
+    // sums from zero and stores every cell (materialize_zero writes the zeros for
+    // every other path first).
+    bool src_zero = false;
     int pins = 0;  // live graphs, batches and adjoints using this field
 };
 
@@ -218,6 +223,8 @@ inline double substep_dt(int prec, double dt_fluid, int nsub) {
 template <class T>
 Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd);  // st_api.cpp
 int merge_source(st_field *f);
+// Writes a pending zero of the source (st_field::src_zero) to memory.
+int materialize_zero(st_field *f);
 // Timed region of one call on the context stream: start() records the start event of the
 // next ring slot, stop() its end event and launch count and makes it the newest slot.
 hipError_t timer_start(st_ctx *ctx, hipStream_t s);

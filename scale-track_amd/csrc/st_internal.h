@@ -81,6 +81,7 @@ struct PassArgs {
     int32_t tile_cells;  // (bd[0]+2)*(bd[1]+2)*(bd[2]+2): a brick with a halo of one cell
     int32_t nsteps;
     int32_t flush;   // 0 owner, 1 atomic
+    int32_t zero_src;  // owner flush: `src` counts as zero (not read); every own cell is stored
     uint32_t np;
     // steps over a sorted cloud's unsorted tail: the first parcel is read from
     // here (the brick ranges' end, brick_start[nbricks]); nullptr otherwise

@@ -562,7 +562,9 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     // memory round trip.  Only this workgroup touches its own cells of `src` during
     // the launch, so the value read here is the one the flush overwrites.  A sparse
     // brick keeps zero-started sums and updates only the cells that took a deposit.
-    const bool seeded = a.flush == 0 && p1 - p0 >= kSeedDensity * ex * ey * ez;
+    // With zero_src the source counts as zero: every brick starts its own cells' sums
+    // from zero without reading `src` and stores all of them, deposits or not.
+    const bool seeded = a.flush == 0 && (a.zero_src != 0 || p1 - p0 >= kSeedDensity * ex * ey * ez);
 
     // Stage the tile's velocities and clear its sums, kStage cells per thread at a
     // time with every load issued before the first LDS write (one memory round trip per
@@ -593,7 +595,7 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             u[k][1] = q[1];
             u[k][2] = q[2];
             o[k][0] = o[k][1] = o[k][2] = 0.0;
-            if (own[k]) {
+            if (own[k] && !a.zero_src) {
                 const double *r = a.src + 3u * c;
                 o[k][0] = r[0];
                 o[k][1] = r[1];
