@@ -160,12 +160,14 @@ int st_field_destroy(st_field *field);
 /* U: host AoS xyz, ncell*3 elements of elem_bytes (4 or 8) each; converted to the
  * field's precision. */
 int st_field_upload_U(st_field *field, const void *U, int32_t elem_bytes);
-/* UTrans = 0 (asyncParallelTracking.jl:1064-1067, syncSerialTracking.jl:553). */
+/* UTrans = 0 (asyncParallelTracking.jl:1064-1067, syncSerialTracking.jl:553).  The
+ * zeros are stored by the next step's first sorted owner-flush pass, which writes every
+ * cell; every other call that uses the source stores them first. */
 int st_field_zero_source(st_field *field);
-/* Completes UTrans on the device: adds the sums of deposits made outside their brick
- * (ST_FLUSH_OWNER) into it, queued on the context stream, without a host copy.  For a
- * caller that reads the source from device memory; st_field_download_source does it
- * itself. */
+/* Completes UTrans on the device: stores a pending zero and adds the sums of deposits
+ * made outside their brick (ST_FLUSH_OWNER) into it, queued on the context stream,
+ * without a host copy.  For a caller that reads the source from device memory;
+ * st_field_download_source does it itself. */
 int st_field_merge_source(st_field *field);
 /* UTrans: host AoS xyz, ncell*3 elements of elem_bytes (8 = the FP64 sums, 4 =
  * rounded to float for a single-precision caller). */
