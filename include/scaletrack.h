@@ -302,8 +302,9 @@ int st_adjoint_gradients(st_adjoint *adj, double *grad_d, double *grad_rho, doub
  * by DMA at the bus's rate (pageable arrays are staged through the runtime's bounce
  * buffers at a fraction of it).  Allocating pins the pages and costs more than one
  * transfer of the same size: allocate once and reuse.  st_host_alloc returns
- * ST_E_INVALID for bytes < 1 or a null out, ST_E_NOMEM when the allocation fails;
- * st_host_free(NULL) does nothing. */
+ * ST_E_INVALID for bytes < 1 or a null out, ST_E_NOMEM when the memory cannot be
+ * allocated and pinned, ST_E_DEVICE without a usable device; st_host_free(NULL) does
+ * nothing. */
 int st_host_alloc(int64_t bytes, void **out);
 int st_host_free(void *ptr);
 

@@ -1325,8 +1325,8 @@ int st_host_alloc(int64_t bytes, void **out) {
     const hipError_t e = hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault);
     if (e != hipSuccess) {
         *out = nullptr;
-        return fail(ST_E_NOMEM, "page-locked host allocation of %lld B: %s", (long long)bytes,
-                    hipGetErrorString(e));
+        return fail(e == hipErrorOutOfMemory ? ST_E_NOMEM : ST_E_DEVICE, "page-locked host allocation of %lld B: %s",
+                    (long long)bytes, hipGetErrorString(e));
     }
     return ST_OK;
 }

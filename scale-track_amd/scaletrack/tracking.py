@@ -44,9 +44,10 @@ class _PinnedBlock:
         self.__array_interface__ = {"shape": shape, "typestr": dt.str, "data": (p.value, False), "version": 3}
 
     def __del__(self):
-        if getattr(self, "ptr", None) is not None and self.ptr.value:
-            lib.st_host_free(self.ptr)
-            self.ptr = None
+        ptr = getattr(self, "ptr", None)
+        if ptr is not None and ptr.value and lib is not None:  # lib is gone at interpreter exit
+            lib.st_host_free(ptr)
+        self.ptr = None
 
 
 def pinned_empty(shape, dtype=np.float64):

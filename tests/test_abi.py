@@ -88,6 +88,13 @@ def test_no_device_is_a_clean_error():
     r = _lib.lib.st_ctx_create(0, C.byref(h))
     assert r == _lib.ST_E_DEVICE and not h.value
     assert _lib.lib.st_last_error()
+    # page-locked host memory needs the device runtime as well
+    p = C.c_void_p()
+    assert _lib.lib.st_host_alloc(64, C.byref(p)) == _lib.ST_E_DEVICE and not p.value
+    import scaletrack as stk
+    with pytest.raises(_lib.ScaleTrackError) as ei:
+        stk.pinned_empty(8)
+    assert ei.value.status == _lib.ST_E_DEVICE
 
 
 def test_error_raises_with_message():
