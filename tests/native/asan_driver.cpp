@@ -224,6 +224,24 @@ void adjoint(st_ctx *ctx) {
         OK(st_adjoint_set_source_cotangent(a, target.data()));
         OK(st_adjoint_backward(a));
         OK(st_adjoint_gradients(a, gd.data(), nullptr, &gcd, nullptr, nullptr));
+        // page-locked host arrays for the target and the gradients, a fresh forward
+        // with only the velocity cotangent seeded (the position one reads as zero)
+        double *pt = nullptr, *pg = nullptr;
+        OK(st_host_alloc((int64_t)(nc * 3 * sizeof(double)), (void **)&pt));
+        OK(st_host_alloc((int64_t)(5 * n * sizeof(double)), (void **)&pg));  // d, rho, x0
+        for (size_t k = 0; k < nc * 3; ++k) pt[k] = target[k];
+        OK(st_adjoint_forward(a, 4e-3));
+        OK(st_adjoint_source_misfit(a, pt, &loss));
+        OK(st_adjoint_set_state_cotangent(a, nullptr, vbar.data()));
+        OK(st_adjoint_backward(a));
+        OK(st_adjoint_gradients(a, pg, pg + n, &gcd, pg + 2 * n, nullptr));
+        for (int64_t k = 2 * n; k < 5 * n; ++k)
+            if (pg[k] != 0.0) {  // dL/dx0 is the position cotangent, which was not given
+                std::fprintf(stderr, "asan_driver: dL/dx0 not zero without a position cotangent\n");
+                std::exit(1);
+            }
+        OK(st_host_free(pt));
+        OK(st_host_free(pg));
         OK(st_adjoint_destroy(a));
     }
     OK(st_cloud_destroy(c));
@@ -259,6 +277,9 @@ void refusals(st_ctx *ctx) {
     expect(st_evolve(ctx, c, f, 1e-3, 10, &p, nullptr), ST_E_INVALID, "st_evolve with block 100");
     expect(st_cloud_inject(c, 5, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr),
            ST_E_INVALID, "st_cloud_inject with null arrays");
+    void *hp = nullptr;
+    expect(st_host_alloc(0, &hp), ST_E_INVALID, "st_host_alloc of 0 bytes");
+    OK(st_host_free(nullptr));
     OK(st_cloud_destroy(c));
     OK(st_field_destroy(f));
 }
