@@ -290,7 +290,13 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
     T nz = pz + nvz * ph.dt;
     double s0x = (double)vx, s0y = (double)vy, s0z = (double)vz;  // :1289
     bool exited = false;
-    if (BND != kPeriodic) {
+    // Walls and open planes: only a new position outside [0, N) on some axis can bounce
+    // or leave (a bounce needs an index out of range), so the per-axis side tests skip
+    // the cell lookup, the fold and the same-cell test for a position inside; outside,
+    // the reference's steps run unchanged.
+    if (BND != kPeriodic && (axis_side((double)nx, g.o[0], g.rd[0], g.n[0]) |
+                             axis_side((double)ny, g.o[1], g.rd[1], g.n[1]) |
+                             axis_side((double)nz, g.o[2], g.rd[2], g.n[2])) != 0) {
         Cell nc;
         locate<T, false>(g, nx, ny, nz, I, nc);  // :1303
         if (BND == kChannel && (uint32_t)I[0] >= (uint32_t)g.n[0]) {
@@ -312,7 +318,7 @@ __device__ __forceinline__ void step_parcel_one(const Geo &g, const Phys<T> &ph,
                 s0z = s0z - 2.0 * (double)vz;
             }
         }
-    } else {
+    } else if (BND == kPeriodic) {
         // periodic extension: shift each axis that left [0, N) by one period
         const int s0 = axis_side((double)nx, g.o[0], g.rd[0], g.n[0]);
         const int s1 = axis_side((double)ny, g.o[1], g.rd[1], g.n[1]);
