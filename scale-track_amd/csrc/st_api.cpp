@@ -433,10 +433,10 @@ int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, c
     // A pending zero of the source (st_field_zero_source) is taken by the first launch
     // when that is one sorted owner-flush pass over the bricks: every brick starts its
     // own cells' sums from zero and stores all of them.  Every other launch reads `src`,
-    // so the zeros are written first.  A capture never meets one: the capture entry
-    // points write it before they begin.
-    const bool lazy = f->src_zero && !capturing && p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER &&
-                      group == 1;
+    // so the zeros are written first.  Under capture a pending zero is the captured
+    // step's own (st_graph_capture writes any earlier one before it begins), and the
+    // captured first pass then takes it on every replay.
+    const bool lazy = f->src_zero && p.method == ST_METHOD_SORTED && p.flush == ST_FLUSH_OWNER && group == 1;
     if (f->src_zero && !lazy) {
         if (capturing) return fail(ST_E_STATE, "a pending zero of the source inside a capture");
         ST_TRY(materialize_zero(f));
@@ -1222,7 +1222,12 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
     }
     int r = ST_OK;
     if (zero_source) {
-        e = st::launch_fill_f64(f->src, 0.0, (size_t)f->ncell * 3, s);
+        // with the owner flush and one pass per launch the captured first pass takes the
+        // zero (no fill node); the stray sums are cleared by a fill kernel either way
+        if (g->owner_flush && p.passes_per_launch <= 1)
+            f->src_zero = true;
+        else
+            e = st::launch_fill_f64(f->src, 0.0, (size_t)f->ncell * 3, s);
         if (e == hipSuccess && g->owner_flush) e = st::launch_fill_f64(f->stray, 0.0, (size_t)f->ncell * 3, s);
         if (e != hipSuccess) r = fail(ST_E_DEVICE, "capture fill: %s", hipGetErrorString(e));
     }
@@ -1232,6 +1237,7 @@ int st_graph_capture(st_ctx *ctx, st_cloud *c, st_field *f, double dt_fluid, int
                 : run_passes<float>(ctx, c, f, substep_dt(c->prec, dt_fluid, nsub), nsub, p, true, sb, true);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(s, &graph);
+    f->src_zero = false;  // a zero the capture set belongs to the replays, not to the field
     if (r != ST_OK || e != hipSuccess) {
         if (graph) hipGraphDestroy(graph);
         delete g;
