@@ -7,6 +7,7 @@
 #include "st_handles.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -14,6 +15,7 @@
 #include <string>
 #include <vector>
 #include <new>
+#include <thread>
 
 namespace st {
 namespace host {
@@ -78,30 +80,67 @@ void quiesce_ctx(st_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
 }
 
-bool finite_all(const void *p, int64_t n, size_t es) {
-    if (es == 8) {
-        const double *a = (const double *)p;
-        for (int64_t i = 0; i < n; ++i)
-            if (!std::isfinite(a[i])) return false;
-    } else {
-        const float *a = (const float *)p;
-        for (int64_t i = 0; i < n; ++i)
-            if (!std::isfinite(a[i])) return false;
+// Runs body(lo, hi) over [0, n), split over up to kHostThreads host threads when n is
+// large: the argument checks and staging copies of big uploads and injections run on
+// the caller's thread, often while the device waits for them.  Each call of body
+// touches only its own range.  If a thread cannot be started its range runs here.
+constexpr int64_t kHostThreads = 8;
+constexpr int64_t kHostChunk = int64_t(1) << 16;
+
+template <class F>
+void host_parallel(int64_t n, const F &body) {
+    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+    const int64_t nt = std::min<int64_t>(std::min<int64_t>(kHostThreads, hw), n / kHostChunk);
+    if (nt < 2) {
+        body((int64_t)0, n);
+        return;
     }
-    return true;
+    const int64_t step = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    th.reserve((size_t)nt);
+    for (int64_t t = 1; t < nt; ++t) {
+        const int64_t lo = t * step, hi = std::min(n, lo + step);
+        if (lo >= hi) continue;
+        try {
+            th.emplace_back([&body, lo, hi] { body(lo, hi); });
+        } catch (...) {
+            body(lo, hi);
+        }
+    }
+    body((int64_t)0, std::min(n, step));
+    for (std::thread &x : th) x.join();
+}
+
+bool finite_all(const void *p, int64_t n, size_t es) {
+    std::atomic<bool> ok{true};
+    host_parallel(n, [&](int64_t lo, int64_t hi) {
+        bool good = true;  // no early exit: the loops vectorise
+        if (es == 8) {
+            const double *a = (const double *)p;
+            for (int64_t i = lo; i < hi; ++i) good &= std::isfinite(a[i]);
+        } else {
+            const float *a = (const float *)p;
+            for (int64_t i = lo; i < hi; ++i) good &= std::isfinite(a[i]);
+        }
+        if (!good) ok.store(false, std::memory_order_relaxed);
+    });
+    return ok.load();
 }
 
 bool positive_all(const void *p, int64_t n, size_t es) {
-    if (es == 8) {
-        const double *a = (const double *)p;
-        for (int64_t i = 0; i < n; ++i)
-            if (!(a[i] > 0)) return false;
-    } else {
-        const float *a = (const float *)p;
-        for (int64_t i = 0; i < n; ++i)
-            if (!(a[i] > 0)) return false;
-    }
-    return true;
+    std::atomic<bool> ok{true};
+    host_parallel(n, [&](int64_t lo, int64_t hi) {
+        bool good = true;
+        if (es == 8) {
+            const double *a = (const double *)p;
+            for (int64_t i = lo; i < hi; ++i) good &= a[i] > 0;
+        } else {
+            const float *a = (const float *)p;
+            for (int64_t i = lo; i < hi; ++i) good &= a[i] > 0;
+        }
+        if (!good) ok.store(false, std::memory_order_relaxed);
+    });
+    return ok.load();
 }
 
 Geo make_geo(const st_mesh &m) {
@@ -319,16 +358,20 @@ int64_t cells_outside(const st_cloud *c, const void *x, const void *y, const voi
     }
     const Geo g = make_geo(m);
     const T *p[3] = {(const T *)x, (const T *)y, (const T *)z};
-    int64_t bad = 0;
-    for (int64_t i = 0; i < k; ++i) {
-        bool out = false;
-        for (int a = 0; a < 3; ++a) {
-            const double v = std::floor(((double)p[a][i] - g.o[a]) * g.rd[a]);
-            out |= !(v >= 0.0 && v < (double)g.n[a]);
+    std::atomic<int64_t> bad{0};
+    host_parallel(k, [&](int64_t lo, int64_t hi) {
+        int64_t b = 0;
+        for (int64_t i = lo; i < hi; ++i) {
+            bool out = false;
+            for (int a = 0; a < 3; ++a) {
+                const double v = std::floor(((double)p[a][i] - g.o[a]) * g.rd[a]);
+                out |= !(v >= 0.0 && v < (double)g.n[a]);
+            }
+            b += out;
         }
-        bad += out;
-    }
-    return bad;
+        bad.fetch_add(b, std::memory_order_relaxed);
+    });
+    return bad.load();
 }
 
 int range_check(st_cloud *c, st_field *f) {
@@ -939,7 +982,12 @@ int st_cloud_inject(st_cloud *c, int64_t k, const void *d, const void *x, const 
             return fail(ST_E_NOMEM, "cannot stage %lld injected parcels in pinned host memory", (long long)k);
         c->stage_cap = part * nf;
     }
-    for (int f = 0; f < nf; ++f) std::memcpy((char *)c->stage + (size_t)f * part, src[f], part);
+    const size_t es = c->es;
+    host_parallel(k, [&](int64_t lo, int64_t hi) {
+        for (int f = 0; f < nf; ++f)
+            std::memcpy((char *)c->stage + (size_t)f * part + (size_t)lo * es, (const char *)src[f] + (size_t)lo * es,
+                        (size_t)(hi - lo) * es);
+    });
     void *dstage = nullptr;
     ST_HIP(hipHostGetDevicePointer(&dstage, c->stage, 0));
     ST_TRY(copy_injected(c, dstage, c->n, k));
