@@ -248,6 +248,43 @@ void adjoint(st_ctx *ctx) {
     OK(st_field_destroy(f));
 }
 
+// An upload and an injection large enough that the library's host checks and staging
+// copy run on several threads.
+void threaded(st_ctx *ctx) {
+    const st_mesh m{{32, 16, 16}, {0.0, 0.0, 0.0}, {2.0, 1.0, 1.0}};
+    const int64_t n = 300000, k = 200000;
+    std::vector<double> U = taylor_green(m);
+    st_field *f = nullptr;
+    st_cloud *c = nullptr;
+    OK(st_field_create(ctx, &m, ST_FP32, &f));
+    OK(st_field_upload_U(f, U.data(), 8));
+    OK(st_cloud_create(ctx, ST_FP32, n, 1e-3, 1000.0, 1.0, 0.0, 0, &c));
+    Host<float> h;
+    h.fill(n, m, 31);
+    OK(st_cloud_upload(c, h.d.data(), h.x.data(), h.y.data(), h.z.data(), h.u.data(), h.v.data(), h.w.data(),
+                       nullptr));
+    st_params p;
+    st_params_default(&p);
+    p.boundary = ST_CHANNEL;
+    p.substeps_per_pass = 1;
+    OK(st_field_zero_source(f));
+    OK(st_evolve(ctx, c, f, 1e-3, 10, &p, nullptr));
+    Host<float> inj;
+    inj.fill(k, m, 32);
+    OK(st_cloud_reserve(c, n + k));
+    OK(st_cloud_inject(c, k, inj.d.data(), inj.x.data(), inj.y.data(), inj.z.data(), inj.u.data(), inj.v.data(),
+                       inj.w.data(), nullptr));
+    inj.x[k - 1] = 3.0f;  // outside the mesh: refused before anything is appended
+    expect(st_cloud_inject(c, k, inj.d.data(), inj.x.data(), inj.y.data(), inj.z.data(), inj.u.data(), inj.v.data(),
+                           inj.w.data(), nullptr),
+           ST_E_RANGE, "st_cloud_inject with a parcel outside the mesh");
+    OK(st_field_zero_source(f));
+    OK(st_evolve(ctx, c, f, 1e-3, 10, &p, nullptr));
+    OK(st_ctx_synchronize(ctx));
+    OK(st_cloud_destroy(c));
+    OK(st_field_destroy(f));
+}
+
 void refusals(st_ctx *ctx) {
     const st_mesh m{{6, 6, 6}, {0, 0, 0}, {1, 1, 1}};
     st_field *f = nullptr;
@@ -308,6 +345,7 @@ int main() {
     forms<float>(ctx, ST_FP32);
     batch(ctx);
     adjoint(ctx);
+    threaded(ctx);
     refusals(ctx);
     // a destroyed context stays alive behind a cloud made on it, and refuses work
     st_ctx *c2 = nullptr;

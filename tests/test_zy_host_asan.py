@@ -21,7 +21,12 @@ SUPP = os.path.join(HERE, "native", "lsan.supp")
 def test_host_code_under_address_sanitizer(leaks):
     assert os.path.exists(DRIVER), "tests/native/asan_driver is not built (make -C tests/native)"
     env = dict(os.environ)
-    env["ASAN_OPTIONS"] = "detect_leaks=1" if leaks else "detect_leaks=0"
+    # A quarantine large enough that nothing freed is recycled while the HIP runtime tears
+    # itself down at exit: recycling a chunk of the runtime's pinned host memory after the
+    # runtime has unloaded trips a CHECK inside the sanitizer's own allocator (the default
+    # 256 MB fills up once the driver's larger clouds have been freed).  A larger
+    # quarantine keeps freed memory poisoned for longer.
+    env["ASAN_OPTIONS"] = ("detect_leaks=1" if leaks else "detect_leaks=0") + ":quarantine_size_mb=2048"
     if leaks:
         env["LSAN_OPTIONS"] = f"suppressions={SUPP}"
     r = subprocess.run([DRIVER], env=env, capture_output=True, text=True, timeout=240)
