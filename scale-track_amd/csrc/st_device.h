@@ -169,7 +169,7 @@ struct Quot<double> {
         ry = fma(-fma(qy, den, -ay), r, qy);
         rz = fma(-fma(qz, den, -az), r, qz);
         const bool huge = !(fmax(fmax(fabs(qx), fabs(qy)), fabs(qz)) <= 0x1p1000);
-        if (tiny(qx) | tiny(qy) | tiny(qz) | huge) {
+        if ((int)tiny(qx) | (int)tiny(qy) | (int)tiny(qz) | (int)huge) {  // one mask, no branches
             rx = ax / den;
             ry = ay / den;
             rz = az / den;
