@@ -36,10 +36,12 @@ struct st_batch {
     size_t lds = 0;
     double dt_sub = -1.0;            // substep the device table holds
     int pins = 0;                    // live graphs of this batch
-    st::FillItem *fills = nullptr;   // device: every field's src, then (owner flush) stray
+    st::FillItem *fills = nullptr;   // device: (owner flush) every field's stray, then every src
     uint32_t *fill_chunk0 = nullptr; // device [nfill + 1]
     int32_t nfill = 0;
     uint32_t fill_chunks = 0;
+    int32_t nfill_stray = 0;         // the stray items, a prefix of the table
+    uint32_t fill_chunks_stray = 0;
     bool pinned_members = false;     // the clouds and fields are pinned by this batch
 };
 
@@ -151,11 +153,10 @@ int sort_all(st_batch *b) {
     return ST_OK;
 }
 
-int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
+// take_zero: every field's source counts as zero in the first pass, whose bricks then
+// store all their own cells (owner flush only); otherwise `src` holds what it holds.
+int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing, bool take_zero) {
     ST_TRY(table_for(b, dt_sub));
-    // the batch kernel reads `src` (under capture the zeros were written before it began)
-    if (!capturing)
-        for (st_field *f : b->fields) ST_TRY(materialize_zero(f));
     hipStream_t s = b->ctx->stream;
     const int per = b->p.substeps_per_pass > 0 ? b->p.substeps_per_pass : nsub;
     if (!capturing) ST_HIP(timer_start(b->ctx, s));
@@ -174,11 +175,11 @@ int queue_passes(st_batch *b, double dt_sub, int nsub, bool capturing) {
             if (b->prec == ST_FP64)
                 ST_HIP(st::launch_pass_batch<double>((const st::PassArgs<double> *)b->descs + c0, b->item0 + c0, nc,
                                                      ns, stats, b->counter, periodic, g, b->p.block, b->grid[g],
-                                                     b->lds, s));
+                                                     b->lds, take_zero && done == 0, s));
             else
                 ST_HIP(st::launch_pass_batch<float>((const st::PassArgs<float> *)b->descs + c0, b->item0 + c0, nc,
                                                     ns, stats, b->counter, periodic, g, b->p.block, b->grid[g],
-                                                    b->lds, s));
+                                                    b->lds, take_zero && done == 0, s));
         }
         ++launches;  // passes: the kernel time per pass spans both launches
     }
@@ -297,15 +298,18 @@ int st_batch_create(st_ctx *ctx, int32_t n, st_cloud *const *clouds, st_field *c
     if (e == hipSuccess) e = hipMalloc((void **)&b->item0, (size_t)(n + 1) * sizeof(int32_t));
     if (e == hipSuccess) e = hipMalloc((void **)&b->targets, (size_t)n * sizeof(st::StatsTarget));
     if (e == hipSuccess) e = hipMalloc((void **)&b->counter, sizeof(unsigned int));
-    // the table that zeroes every field's source (and stray sums) in one launch
+    // the table that zeroes every field's stray sums and source in one launch; its
+    // prefix of stray items alone serves a step whose first pass takes the source's zero
     std::vector<st::FillItem> fills;
-    for (st_field *f : b->fields) fills.push_back({f->src, (size_t)f->ncell * 3});
     if (p.flush == ST_FLUSH_OWNER)
         for (st_field *f : b->fields) fills.push_back({f->stray, (size_t)f->ncell * 3});
+    b->nfill_stray = (int32_t)fills.size();
+    for (st_field *f : b->fields) fills.push_back({f->src, (size_t)f->ncell * 3});
     std::vector<uint32_t> chunk0(fills.size() + 1, 0);
     for (size_t i = 0; i < fills.size(); ++i) chunk0[i + 1] = chunk0[i] + st::fill_many_chunks(fills[i].n);
     b->nfill = (int32_t)fills.size();
     b->fill_chunks = chunk0.back();
+    b->fill_chunks_stray = chunk0[(size_t)b->nfill_stray];
     if (e == hipSuccess) e = hipMalloc((void **)&b->fills, fills.size() * sizeof(st::FillItem));
     if (e == hipSuccess) e = hipMalloc((void **)&b->fill_chunk0, chunk0.size() * sizeof(uint32_t));
     if (e == hipSuccess)
@@ -345,7 +349,16 @@ int st_batch_evolve(st_batch *b, double dt_fluid, int32_t nsub) {
     ST_TRY(check_ctx(b->ctx));
     if (!std::isfinite(dt_fluid) || !(dt_fluid > 0)) return fail(ST_E_INVALID, "dt_fluid must be finite and > 0");
     if (nsub < 1) return fail(ST_E_INVALID, "nsub must be >= 1");
-    return queue_passes(b, substep_dt(b->prec, dt_fluid, nsub), nsub, false);
+    // zeros pending on every field (st_field_zero_source) are taken by the first pass;
+    // otherwise the kernel reads `src`, so pending ones are written first
+    bool take = b->p.flush == ST_FLUSH_OWNER;
+    for (st_field *f : b->fields) take = take && f->src_zero;
+    if (!take)
+        for (st_field *f : b->fields) ST_TRY(materialize_zero(f));
+    ST_TRY(queue_passes(b, substep_dt(b->prec, dt_fluid, nsub), nsub, false, take));
+    if (take)
+        for (st_field *f : b->fields) f->src_zero = false;
+    return ST_OK;
 }
 
 int st_batch_graph_capture(st_batch *b, double dt_fluid, int32_t nsub, int32_t zero_source, st_graph **out) {
@@ -371,11 +384,15 @@ int st_batch_graph_capture(st_batch *b, double dt_fluid, int32_t nsub, int32_t z
         return fail(ST_E_DEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     }
     int r = ST_OK;
-    if (zero_source) {  // every field's source (and stray sums) in one launch
-        e = st::launch_fill_many(b->fills, b->fill_chunk0, b->nfill, b->fill_chunks, s);
+    // with the owner flush the first captured pass takes the sources' zero and only the
+    // stray sums are filled; otherwise every source is filled too, in one launch
+    const bool take = zero_source && b->p.flush == ST_FLUSH_OWNER;
+    if (zero_source && (take ? b->nfill_stray : b->nfill) > 0) {
+        e = take ? st::launch_fill_many(b->fills, b->fill_chunk0, b->nfill_stray, b->fill_chunks_stray, s)
+                 : st::launch_fill_many(b->fills, b->fill_chunk0, b->nfill, b->fill_chunks, s);
         if (e != hipSuccess) r = fail(ST_E_DEVICE, "capture fill: %s", hipGetErrorString(e));
     }
-    if (r == ST_OK) r = queue_passes(b, dt_sub, nsub, true);
+    if (r == ST_OK) r = queue_passes(b, dt_sub, nsub, true, take);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(s, &graph);
     if (r != ST_OK || e != hipSuccess) {

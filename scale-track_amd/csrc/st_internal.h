@@ -133,9 +133,10 @@ int pass_multi_blocks_per_cu(int bnd, int rho, int block, size_t lds);
 // (counter must hold item0[0] at launch); rho: every one of these clouds carries a
 // per-parcel density array (else none does).
 template <class T>
+// zero_src: every cloud's source counts as zero in this pass (PassArgs::zero_src).
 hipError_t launch_pass_batch(const PassArgs<T> *descs, const int32_t *item0, int32_t ncloud, int32_t nsteps,
                              int stats, unsigned int *counter, int periodic, int rho, int block, int grid,
-                             size_t lds, hipStream_t s);
+                             size_t lds, int zero_src, hipStream_t s);
 template <class T>
 int batch_max_blocks_per_cu(int periodic, int rho, int block, size_t lds);
 hipError_t launch_reduce_stats(const StatsTarget &t, hipStream_t s);

@@ -822,7 +822,7 @@ template int pass_multi_blocks_per_cu<double>(int, int, int, size_t);
 template <class T, int BND, bool RHO>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4)))
 k_pass_batch(const PassArgs<T> *__restrict__ descs, const int32_t *__restrict__ item0, int32_t ncloud,
-             int32_t nsteps, int stats, unsigned int *counter) {
+             int32_t nsteps, int stats, unsigned int *counter, int zero_src) {
     extern __shared__ double lds[];
     __shared__ int32_t s_item;
     const int32_t total = item0[ncloud];
@@ -841,7 +841,8 @@ k_pass_batch(const PassArgs<T> *__restrict__ descs, const int32_t *__restrict__ 
                 hi = mid - 1;
         }
         const int32_t c = __builtin_amdgcn_readfirstlane(lo);
-        const PassArgs<T> a = descs[c];
+        PassArgs<T> a = descs[c];
+        a.zero_src = zero_src;
         process_brick<T, BND, RHO>(a, item - item0[c], nsteps, lds, stats ? a.part : nullptr);
         __syncthreads();  // the flush's tile reads are done before the next staging
         if (threadIdx.x == 0) s_item = next;
@@ -853,21 +854,22 @@ k_pass_batch(const PassArgs<T> *__restrict__ descs, const int32_t *__restrict__ 
 template <class T>
 hipError_t launch_pass_batch(const PassArgs<T> *descs, const int32_t *item0, int32_t ncloud, int32_t nsteps,
                              int stats, unsigned int *counter, int periodic, int rho, int block, int grid,
-                             size_t lds, hipStream_t s) {
+                             size_t lds, int zero_src, hipStream_t s) {
     if (periodic && rho)
-        k_pass_batch<T, true, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
+        k_pass_batch<T, true, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter, zero_src);
     else if (periodic)
-        k_pass_batch<T, true, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
+        k_pass_batch<T, true, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter, zero_src);
     else if (rho)
-        k_pass_batch<T, false, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
+        k_pass_batch<T, false, true><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter, zero_src);
     else
-        k_pass_batch<T, false, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter);
+        k_pass_batch<T, false, false><<<grid, block, lds, s>>>(descs, item0, ncloud, nsteps, stats, counter,
+                                                               zero_src);
     return hipGetLastError();
 }
 template hipError_t launch_pass_batch<float>(const PassArgs<float> *, const int32_t *, int32_t, int32_t, int,
-                                             unsigned int *, int, int, int, int, size_t, hipStream_t);
+                                             unsigned int *, int, int, int, int, size_t, int, hipStream_t);
 template hipError_t launch_pass_batch<double>(const PassArgs<double> *, const int32_t *, int32_t, int32_t, int,
-                                              unsigned int *, int, int, int, int, size_t, hipStream_t);
+                                              unsigned int *, int, int, int, int, size_t, int, hipStream_t);
 
 template <class T>
 int batch_max_blocks_per_cu(int periodic, int rho, int block, size_t lds) {
