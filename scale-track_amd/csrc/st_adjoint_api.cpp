@@ -146,12 +146,30 @@ int st_adjoint_forward(st_adjoint *a, double dt_fluid) {
     ST_TRY(st_field_zero_source(f));
     a->dt_sub = substep_dt(ST_FP64, dt_fluid, a->nsub);
     const uint32_t n = a->n;
+    // Sorted method: segment s steps its state from checkpoint s into checkpoint s + 1
+    // (one launch each), so no segment copies a state it can write where it is needed;
+    // the first reads the cloud's arrays and also writes checkpoint 0, the last writes
+    // the cloud's arrays.
+    const st::Soa<double> cloud = soa<double>(c, c->cur);
+    auto ck_soa = [&](int32_t k) {
+        st::Soa<double> q = cloud;  // d and rho stay the cloud's
+        double *b = a->ck + (size_t)k * 6 * n;
+        q.x = b;
+        q.y = b + n;
+        q.z = b + 2 * (size_t)n;
+        q.u = b + 3 * (size_t)n;
+        q.v = b + 4 * (size_t)n;
+        q.w = b + 5 * (size_t)n;
+        return q;
+    };
     for (int32_t seg = 0; seg < a->nseg; ++seg) {
         double *ck = a->ck + (size_t)seg * 6 * n;
         const int32_t ks = std::min(a->K, a->nsub - seg * a->K);
         if (p.method == ST_METHOD_SORTED) {
-            // the segment's first pass writes the checkpoint from the state it loads
-            ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false, sb, false, ck));
+            const st::Soa<double> in = seg == 0 ? cloud : ck_soa(seg);
+            const st::Soa<double> out = seg == a->nseg - 1 ? cloud : ck_soa(seg + 1);
+            ST_TRY(run_passes<double>(a->ctx, c, f, a->dt_sub, ks, p, false, sb, false, seg == 0 ? ck : nullptr, &in,
+                                      &out));
         } else {
             st::Soa<double> cur = soa<double>(c, c->cur);
             const double *fields[6] = {cur.x, cur.y, cur.z, cur.u, cur.v, cur.w};

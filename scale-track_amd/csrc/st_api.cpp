@@ -443,11 +443,18 @@ int cloud_stats(st_cloud *c, int32_t nslots, StatsBuf &sb) {
 
 template <class T>
 int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p,
-               bool capturing, const StatsBuf &sb, bool device_resort, T *ck) {
+               bool capturing, const StatsBuf &sb, bool device_resort, T *ck, const Soa<T> *in,
+               const Soa<T> *out) {
     hipStream_t s = ctx->stream;
     if (!capturing) observe_step(c);
+    // in/out: the adjoint's forward steps a segment from one set of state arrays into
+    // another (one sorted launch, FP64); every other step works in place
+    if ((in || out) && (!ck && !out)) return fail(ST_E_INVALID, "state arrays given without a checkpointing pass");
+    if ((in || out) && (p.method != ST_METHOD_SORTED || (p.substeps_per_pass > 0 && p.substeps_per_pass < nsub)))
+        return fail(ST_E_INVALID, "separate state arrays need one sorted launch for the segment");
     PassArgs<T> a{};
-    a.s = soa<T>(c, c->cur);
+    a.s = in ? *in : soa<T>(c, c->cur);
+    a.out = out ? *out : a.s;
     a.U = (const T *)f->U;
     a.src = f->src;
     a.stray = f->stray;
@@ -645,9 +652,9 @@ Phys<T> make_phys(const st_cloud *c, const st_field *f, double dt_sub, double cd
 template Phys<float> make_phys<float>(const st_cloud *, const st_field *, double, double);
 template Phys<double> make_phys<double>(const st_cloud *, const st_field *, double, double);
 template int run_passes<float>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool,
-                               const StatsBuf &, bool, float *);
+                               const StatsBuf &, bool, float *, const Soa<float> *, const Soa<float> *);
 template int run_passes<double>(st_ctx *, st_cloud *, st_field *, double, int, const st_params &, bool,
-                                const StatsBuf &, bool, double *);
+                                const StatsBuf &, bool, double *, const Soa<double> *, const Soa<double> *);
 
 }  // namespace host
 }  // namespace st

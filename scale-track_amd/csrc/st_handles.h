@@ -203,7 +203,8 @@ bool sort_buffers_fit(const st_cloud *c);
 // gated on the share of parcels found outside their brick exceeding p.resort_fraction.
 template <class T>
 int run_passes(st_ctx *ctx, st_cloud *c, st_field *f, double dt_sub, int nsub, const st_params &p, bool capturing,
-               const StatsBuf &sb, bool device_resort, T *ck = nullptr);
+               const StatsBuf &sb, bool device_resort, T *ck = nullptr, const st::Soa<T> *in = nullptr,
+               const st::Soa<T> *out = nullptr);
 // Looks up the device's copy of the no-allocation entry point's statistics scratch
 // (st_noalloc.hip); called by st_ctx_create with the device current.
 int noalloc_setup(int device);

@@ -73,6 +73,8 @@ struct PassArgs {
     Geo g;
     Phys<T> ph;
     double *part;    // per-workgroup statistics slots, or nullptr (no statistics this pass)
+    Soa<T> out;      // where the checkpointing pass stores the new state (x..w; may be s itself);
+                     // every other pass stores into s
     T *ck;           // checkpoint of the state each parcel starts the pass with (6 arrays of np,
                      // device order), or nullptr; sorted method, FP64 only
     int32_t nbricks;

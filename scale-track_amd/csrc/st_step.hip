@@ -638,6 +638,8 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
     //    instead of waiting for them.
     // (A second parcel in flight per thread measured no faster and costs the FP64 and
     // batch kernels occupancy.)
+    // the checkpointing pass may store the new state into other arrays than it loads
+    const Soa<T> so = CKPT && a.out.x != nullptr ? a.out : a.s;
     if (p1 > p0) {
         __builtin_amdgcn_s_waitcnt(0);
         auto ahead = [&](int32_t q) { return min((uint32_t)(q + bs), last); };
@@ -645,8 +647,8 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
             const bool stats = part != nullptr;
             auto one = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
                 const uint32_t q = ahead(p);
-                if (CKPT) save_checkpoint(a.ck, a.np, (uint32_t)p, cur);
-                step_parcel_one<T, BND, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, env, box, stats,
+                if (CKPT && a.ck) save_checkpoint(a.ck, a.np, (uint32_t)p, cur);
+                step_parcel_one<T, BND, RHO>(a.g, a.ph, so, cur, (uint32_t)p, env, box, stats,
                                                   [&] { load_parcel<T, RHO>(a.s, q, a.ph.rho, fill); });
                 p += bs;
             };
@@ -659,8 +661,8 @@ __device__ __forceinline__ void process_brick(const PassArgs<T> &a, int32_t bric
         } else {
             auto many = [&](const ParcelIn<T> &cur, ParcelIn<T> &fill) {
                 const uint32_t q = ahead(p);
-                if (CKPT) save_checkpoint(a.ck, a.np, (uint32_t)p, cur);
-                step_parcel<T, BND, RHO>(a.g, a.ph, a.s, cur, (uint32_t)p, nsteps, env, box,
+                if (CKPT && a.ck) save_checkpoint(a.ck, a.np, (uint32_t)p, cur);
+                step_parcel<T, BND, RHO>(a.g, a.ph, so, cur, (uint32_t)p, nsteps, env, box,
                                               [&] { load_parcel<T, RHO>(a.s, q, a.ph.rho, fill); });
                 p += bs;
             };
@@ -902,7 +904,7 @@ template <class T>
 hipError_t launch_pass_sorted(const PassArgs<T> &a, int bnd, int block, hipStream_t s) {
     const size_t lds = (size_t)a.tile_cells * 3 * (sizeof(double) + sizeof(T));
     if constexpr (sizeof(T) == 8) {
-        if (a.ck) {  // the adjoint's forward: wall and periodic forms only
+        if (a.ck || (a.out.x && a.out.x != a.s.x)) {  // the adjoint's forward: wall and periodic forms only
             with_forms(bnd == kPeriodic ? kPeriodic : kWall, a.s.rho != nullptr, [&](auto B, auto R) {
                 k_pass_sorted<T, decltype(B)::value, decltype(R)::value, true><<<a.nbricks, block, lds, s>>>(a);
             });
